@@ -1,0 +1,105 @@
+/*
+ * notorious_fft_hip.h -- extended C ABI of the MI355X (gfx950) FFT engine.
+ *
+ * Everything here is new relative to the reference library, whose API is
+ * one synchronous transform per call on host memory (src/07_legacy.h:29-57).
+ * This header adds, with plain pointers and sizes only:
+ *   - device-resident plans ("device-aux"): tables and scratch in HBM;
+ *   - uniform batches of contiguous transforms on a caller stream;
+ *   - an explicit normalisation flag;
+ *   - vector-Jacobian products (adjoints) of every transform;
+ *   - status codes with a thread-local message.
+ * Library: libnotorious_fft_hip.so (both precisions; the precision is a plan
+ * argument).  Streams are hipStream_t passed as void*; NULL is the default
+ * stream.  Calls enqueue work and return; synchronise the stream before
+ * reading results.  A plan holds scratch: two executions of one plan must not
+ * overlap in time (the same rule as the reference's aux, src/07_legacy.h:133).
+ */
+#ifndef NOTORIOUS_FFT_HIP_H
+#define NOTORIOUS_FFT_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nfft_plan nfft_plan;
+
+/* status codes */
+enum {
+    NFFT_OK = 0,
+    NFFT_E_INVALID = 1,     /* bad argument (NULL, wrong op for the plan kind) */
+    NFFT_E_SIZE = 2,        /* size not supported (the reference's NULL-aux cases) */
+    NFFT_E_ALLOC = 3,       /* device or host allocation failed */
+    NFFT_E_HIP = 4,         /* a HIP runtime call or kernel launch failed */
+    NFFT_E_UNSUPPORTED = 5  /* valid request outside this build's limits */
+};
+
+/* precision */
+enum { NFFT_F32 = 0, NFFT_F64 = 1 };
+
+/* plan kinds, as the reference's aux families (mkaux_dft / realdft / t2t3 / t4) */
+enum { NFFT_KIND_DFT = 0, NFFT_KIND_REALDFT = 1, NFFT_KIND_T2T3 = 2, NFFT_KIND_T4 = 3 };
+
+/* operations, as the reference's transform entry points */
+enum {
+    NFFT_OP_DFT = 0, NFFT_OP_INVDFT = 1, NFFT_OP_REALDFT = 2, NFFT_OP_INVREALDFT = 3,
+    NFFT_OP_DCT2 = 4, NFFT_OP_DST2 = 5, NFFT_OP_DCT3 = 6, NFFT_OP_DST3 = 7,
+    NFFT_OP_DCT4 = 8, NFFT_OP_DST4 = 9
+};
+
+/* normalisation of inverse complex transforms:
+ *   REFERENCE  the reference's behaviour: invdft unnormalised for power-of-two
+ *              lengths, scaled by 1/N along every non-power-of-two axis
+ *              (src/05_algorithms.h:315-333);
+ *   NONE       never scaled (the adjoint of dft);
+ *   INVERSE    invdft scaled by 1/prod(N). */
+enum { NFFT_NORM_REFERENCE = 0, NFFT_NORM_NONE = 1, NFFT_NORM_INVERSE = 2 };
+
+/* Create a plan for `batch` contiguous transforms of dims[0..d-1] (row-major,
+ * last index fastest, FFTW layout).  Element counts per transform:
+ *   DFT      in/out prod(dims) complex (interleaved re,im)
+ *   REALDFT  real prod(dims) <-> complex prod(dims[0..d-2]) * (dims[d-1]/2+1)
+ *   T2T3/T4  prod(dims) reals
+ * Size rules follow the reference (src/07_legacy.h:1405-1634): DFT accepts
+ * any N >= 1 per axis; REALDFT, T2T3 and T4 need powers of two. */
+int nfft_plan_create(nfft_plan **plan, int kind, int prec, int d, const int *dims,
+                     long long batch, int norm);
+int nfft_plan_destroy(nfft_plan *plan);
+
+/* Run `op` on device buffers.  in == out is allowed except for REALDFT /
+ * INVREALDFT with batch > 1.  The input is never modified. */
+int nfft_execute(nfft_plan *plan, int op, const void *in, void *out, void *stream);
+
+/* Vector-Jacobian product of `op`: grad_in = J^T grad_out, where complex
+ * gradients use the convention g = dL/dRe + i dL/dIm.  Layouts are those of
+ * the op's output (grad_out) and input (grad_in).
+ *   dft:        DFT^H g  (unnormalised inverse DFT)
+ *   invdft:     DFT g, scaled as the plan's invdft is
+ *   realdft:    invrealdft(g * [1, 1/2, ..., 1/2, 1])
+ *   invrealdft: realdft(g) with bins 0 < k < N/2 doubled
+ *   dct2/dst2:  dct3/dst3 with the DC input doubled
+ *   dct3/dst3:  dct2/dst2 with the DC output halved
+ *   dct4/dst4:  self-adjoint */
+int nfft_vjp(nfft_plan *plan, int op, const void *grad_out, void *grad_in, void *stream);
+
+/* scratch bytes held by the plan in HBM */
+long long nfft_plan_workspace_bytes(const nfft_plan *plan);
+
+/* A short description of the execution strategy ("line", "four-step
+ * 1024x1024", "bluestein M=2048", ...) for diagnostics. */
+const char *nfft_plan_describe(const nfft_plan *plan);
+
+/* Message for the last error on this thread ("" if none). */
+const char *nfft_last_error(void);
+
+/* Device helpers used by the tests and the benchmark: fill `count` values
+ * (reals) with the counter-based SplitMix64 uniform [-1, 1) stream, value i
+ * = U(seed, start + i); prec selects float or double output. */
+int nfft_fill_uniform(void *out, long long count, unsigned long long seed,
+                      unsigned long long start, int prec, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NOTORIOUS_FFT_HIP_H */

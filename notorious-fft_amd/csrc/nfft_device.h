@@ -1,0 +1,287 @@
+// nfft_device.h -- device-side building blocks for the CDNA4 (gfx950) FFT
+// kernels: complex helpers, register radix-2/4/8/16 codelets and the
+// LDS-exchanged Stockham "line" FFT that every transform kernel is built on.
+//
+// A line of L = 2^p points is transformed by NT = L/16 threads (L >= 16), each
+// holding 16 points in VGPRs: reg[q] is point (j + NT*q) of the line, both on
+// entry (input order) and on exit (natural output order).  Stages are radix-16
+// except the first (radix L/16^(S-1) in {2,4,8,16}); between stages the line
+// is exchanged through LDS in Stockham (self-sorting) order, so no
+// bit-reversal pass exists.  Stage twiddles come from a per-L table generated
+// in FP64 on the device (one 16-entry row per butterfly index k).
+//
+// The algorithm replaces the reference's depth-first split-radix DIF
+// (src/05_algorithms.h:358-717) and radix-2 DIT (src/05_algorithms.h:28-287):
+// same transform, different factorisation chosen for 64-wide wavefronts.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <type_traits>
+
+namespace nfft {
+
+template <typename T> struct CxT;
+template <> struct CxT<float> { using V = float2; };
+template <> struct CxT<double> { using V = double2; };
+template <typename T> using Cx = typename CxT<T>::V;
+
+template <typename C> __device__ __forceinline__ C cmk(decltype(C::x) a, decltype(C::x) b)
+{
+    C r;
+    r.x = a;
+    r.y = b;
+    return r;
+}
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { return cmk<C>(a.x + b.x, a.y + b.y); }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { return cmk<C>(a.x - b.x, a.y - b.y); }
+template <typename C> __device__ __forceinline__ C cneg(C a) { return cmk<C>(-a.x, -a.y); }
+template <typename C> __device__ __forceinline__ C cconj(C a) { return cmk<C>(a.x, -a.y); }
+template <typename C> __device__ __forceinline__ C cscale(C a, decltype(C::x) s) { return cmk<C>(a.x * s, a.y * s); }
+template <typename C> __device__ __forceinline__ C cmul(C a, C b)
+{
+    return cmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+template <typename C> __device__ __forceinline__ C cmulc(C a, C b)
+{
+    return cmk<C>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+// a * w where w is a forward twiddle; DIR > 0 uses conj(w).
+template <int DIR, typename C> __device__ __forceinline__ C ctw(C a, C w)
+{
+    if constexpr (DIR < 0) return cmul(a, w);
+    else return cmulc(a, w);
+}
+// multiply by DIR*i  (DIR = -1: forward e^{-i pi/2}; +1: inverse)
+template <int DIR, typename C> __device__ __forceinline__ C mulj(C a)
+{
+    if constexpr (DIR < 0) return cmk<C>(a.y, -a.x);
+    else return cmk<C>(-a.y, a.x);
+}
+
+// cos(pi*m/8), sin(pi*m/8), m = 0..15
+constexpr double kCos16[16] = {
+    1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+    0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+    -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173,
+    0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613};
+constexpr double kSin16[16] = {
+    0.0, 0.38268343236508977173, 0.70710678118654752440, 0.92387953251128675613,
+    1.0, 0.92387953251128675613, 0.70710678118654752440, 0.38268343236508977173,
+    0.0, -0.38268343236508977173, -0.70710678118654752440, -0.92387953251128675613,
+    -1.0, -0.92387953251128675613, -0.70710678118654752440, -0.38268343236508977173};
+
+// a * e^{DIR * 2 pi i * E / R}, E and R compile-time, R divides 16.
+template <int DIR, int R, int E, typename C> __device__ __forceinline__ C twc(C a)
+{
+    constexpr int e = ((E % R) + R) % R;
+    constexpr int m = e * (16 / R);  // angle = pi*m/8
+    using T = decltype(C::x);
+    if constexpr (m == 0) return a;
+    else if constexpr (m == 4) return mulj<DIR>(a);
+    else if constexpr (m == 8) return cneg(a);
+    else if constexpr (m == 12) return mulj<-DIR>(a);
+    else {
+        constexpr double c = kCos16[m];
+        constexpr double s = DIR * kSin16[m];
+        if constexpr (m == 2 || m == 6 || m == 10 || m == 14) {
+            // |c| == |s| == 1/sqrt2: (a.x c - a.y s, a.x s + a.y c) with one product each
+            constexpr T h = (T)0.70710678118654752440;
+            constexpr T sc = c > 0 ? 1 : -1, ss = s > 0 ? 1 : -1;
+            return cmk<C>(h * (sc * a.x - ss * a.y), h * (ss * a.x + sc * a.y));
+        } else {
+            return cmk<C>(a.x * (T)c - a.y * (T)s, a.x * (T)s + a.y * (T)c);
+        }
+    }
+}
+
+template <int I, int N, typename F> __device__ __forceinline__ void sfor(F &&f)
+{
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        sfor<I + 1, N>(f);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Register codelets: v[0..R-1] natural order in and out, X_k = sum v_n w^{nk},
+// w = e^{DIR 2 pi i / R}.
+// ---------------------------------------------------------------------------
+template <int DIR, typename C> __device__ __forceinline__ void dft2(C &a, C &b)
+{
+    C t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <int DIR, typename C> __device__ __forceinline__ void dft4(C &a0, C &a1, C &a2, C &a3)
+{
+    C s0 = cadd(a0, a2), d0 = csub(a0, a2);
+    C s1 = cadd(a1, a3), d1 = mulj<DIR>(csub(a1, a3));
+    a0 = cadd(s0, s1);
+    a2 = csub(s0, s1);
+    a1 = cadd(d0, d1);
+    a3 = csub(d0, d1);
+}
+
+template <int DIR, typename C> __device__ __forceinline__ void dft8(C *v)
+{
+    C e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    C o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    dft4<DIR>(e0, e1, e2, e3);
+    dft4<DIR>(o0, o1, o2, o3);
+    o1 = twc<DIR, 8, 1>(o1);
+    o2 = twc<DIR, 8, 2>(o2);
+    o3 = twc<DIR, 8, 3>(o3);
+    v[0] = cadd(e0, o0);
+    v[4] = csub(e0, o0);
+    v[1] = cadd(e1, o1);
+    v[5] = csub(e1, o1);
+    v[2] = cadd(e2, o2);
+    v[6] = csub(e2, o2);
+    v[3] = cadd(e3, o3);
+    v[7] = csub(e3, o3);
+}
+
+// 16 = 4 x 4: n = 4 n1 + n2, k = k1 + 4 k2.
+template <int DIR, typename C> __device__ __forceinline__ void dft16(C *v)
+{
+    C a[16];
+    sfor<0, 4>([&](auto n2c) {
+        constexpr int n2 = decltype(n2c)::value;
+        C x0 = v[n2], x1 = v[4 + n2], x2 = v[8 + n2], x3 = v[12 + n2];
+        dft4<DIR>(x0, x1, x2, x3);
+        a[n2 * 4 + 0] = x0;
+        a[n2 * 4 + 1] = twc<DIR, 16, n2 * 1>(x1);
+        a[n2 * 4 + 2] = twc<DIR, 16, n2 * 2>(x2);
+        a[n2 * 4 + 3] = twc<DIR, 16, n2 * 3>(x3);
+    });
+    sfor<0, 4>([&](auto k1c) {
+        constexpr int k1 = decltype(k1c)::value;
+        C y0 = a[0 + k1], y1 = a[4 + k1], y2 = a[8 + k1], y3 = a[12 + k1];
+        dft4<DIR>(y0, y1, y2, y3);
+        v[k1] = y0;
+        v[k1 + 4] = y1;
+        v[k1 + 8] = y2;
+        v[k1 + 12] = y3;
+    });
+}
+
+template <int R, int DIR, typename C> __device__ __forceinline__ void dftR(C *v)
+{
+    if constexpr (R == 1) {
+    } else if constexpr (R == 2) dft2<DIR>(v[0], v[1]);
+    else if constexpr (R == 4) dft4<DIR>(v[0], v[1], v[2], v[3]);
+    else if constexpr (R == 8) dft8<DIR>(v);
+    else if constexpr (R == 16) dft16<DIR>(v);
+    else static_assert(R == 16, "unsupported radix");
+}
+
+// ---------------------------------------------------------------------------
+// Stage schedule of an L-point line
+// ---------------------------------------------------------------------------
+__host__ __device__ constexpr int ilog2c(int v) { return v <= 1 ? 0 : 1 + ilog2c(v >> 1); }
+
+template <int L> struct Sched {
+    static_assert(L >= 1 && (L & (L - 1)) == 0, "power of two");
+    static constexpr int LG = ilog2c(L);
+    static constexpr int S = LG <= 4 ? 1 : (LG + 3) / 4;  // stages
+    static constexpr int R = L >= 16 ? 16 : L;            // points per thread
+    static constexpr int NT = L / R;                      // threads per line
+    static constexpr int R0 = L >> (4 * (S - 1));         // first radix
+    static constexpr int radix(int s) { return s == 0 ? R0 : 16; }
+    static constexpr int ns(int s) { return s == 0 ? 1 : R0 << (4 * (s - 1)); }
+    // twiddle table: stage s >= 1 holds ns(s) rows of 16 entries
+    static constexpr int twoff(int s) { return s <= 1 ? 0 : twoff(s - 1) + ns(s - 1) * 16; }
+    static constexpr int TWN = S <= 1 ? 0 : twoff(S);
+};
+
+// LDS padding for line-major layouts: one element per 16 (FP32) / 8 (FP64)
+// keeps the strided Stockham writes of the first stages off a single bank.
+template <typename T> __host__ __device__ constexpr int lpad(int p)
+{
+    return sizeof(T) == 4 ? p + (p >> 4) : p + (p >> 3);
+}
+template <typename T, int L> __host__ __device__ constexpr int lpadlen()
+{
+    return lpad<T>(L - 1) + 1;
+}
+
+// Line-major LDS layout: line b owns [b*PADL, (b+1)*PADL).
+template <typename T, int L> struct LayLine {
+    int base;
+    __device__ __forceinline__ int operator()(int p) const { return base + lpad<T>(p); }
+};
+// Column layout: point p of line b at p*LPB + b (b fastest).
+template <int LPB> struct LayCol {
+    int b;
+    __device__ __forceinline__ int operator()(int p) const { return p * LPB + b; }
+};
+
+// ---------------------------------------------------------------------------
+// The line FFT.  All threads of the block must call it (it contains block
+// barriers when S > 1).  reg[q] <-> point j + NT*q.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int DIR, class Lay>
+__device__ __forceinline__ void line_fft(Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds, const Lay &lay, int j,
+                                         const Cx<T> *__restrict__ tw)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    sfor<0, S::S>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr int Rs = S::radix(s), Ns = S::ns(s), m = S::R / Rs;
+#pragma unroll
+        for (int i = 0; i < m; i++) {
+            const int v = j + S::NT * i;
+            C a[Rs];
+#pragma unroll
+            for (int r = 0; r < Rs; r++) a[r] = reg[i + r * m];
+            if constexpr (s > 0) {
+                const int k = v & (Ns - 1);
+                const C *t = tw + S::twoff(s) + k * 16;
+#pragma unroll
+                for (int r = 1; r < Rs; r++) a[r] = ctw<DIR>(a[r], t[r]);
+            }
+            dftR<Rs, DIR>(a);
+#pragma unroll
+            for (int r = 0; r < Rs; r++) reg[i + r * m] = a[r];
+        }
+        if constexpr (s + 1 < S::S) {
+#pragma unroll
+            for (int i = 0; i < m; i++) {
+                const int v = j + S::NT * i;
+                const int base = (v / Ns) * Ns * Rs + (v & (Ns - 1));
+#pragma unroll
+                for (int r = 0; r < Rs; r++) lds[lay(base + r * Ns)] = reg[i + r * m];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < S::R; q++) reg[q] = lds[lay(j + S::NT * q)];
+            __syncthreads();
+        }
+    });
+}
+
+// Twiddle for the four-step / Bluestein inter-pass products: w_N^{e},
+// e < N, from two FP64 tables (hi: e >> sh, lo: e & mask), product in FP64.
+struct Tw2 {
+    const double2 *hi;
+    const double2 *lo;
+    int sh;
+    int mask;
+};
+__device__ __forceinline__ double2 tw2_get(const Tw2 &t, uint32_t e)
+{
+    double2 h = t.hi[e >> t.sh], l = t.lo[e & t.mask];
+    double2 r;
+    r.x = h.x * l.x - h.y * l.y;
+    r.y = h.x * l.y + h.y * l.x;
+    return r;
+}
+template <typename T> __device__ __forceinline__ Cx<T> to_cx(double2 w)
+{
+    return cmk<Cx<T>>((T)w.x, (T)w.y);
+}
+
+}  // namespace nfft

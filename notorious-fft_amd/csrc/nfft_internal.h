@@ -1,0 +1,151 @@
+// nfft_internal.h -- host-side internals shared by the planner, the C ABI
+// and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "nfft_kernels.h"
+#include "nfft_kernels_rt.h"
+
+namespace nfft {
+
+enum Prec { F32 = 0, F64 = 1 };
+
+inline size_t cx_bytes(int prec) { return prec == F32 ? 8 : 16; }
+inline size_t re_bytes(int prec) { return prec == F32 ? 4 : 8; }
+
+// thread-local error message + status helpers (nfft_plan.cpp)
+void set_error(const std::string &msg);
+void clear_error();
+const char *last_error();
+
+// single-pass limits (LDS: 160 KB per workgroup on gfx950)
+constexpr int kLineMax32 = 16384;
+constexpr int kLineMax64 = 8192;
+inline int line_max(int prec) { return prec == F32 ? kLineMax32 : kLineMax64; }
+// column pass (k_col) and transposed row pass (k_rowT) limits
+inline int col_max(int /*prec*/) { return 4096; }
+inline int rowT_max(int prec) { return prec == F32 ? 2048 : 1024; }
+inline int bluerow_max(int prec) { return prec == F32 ? 4096 : 2048; }
+
+// Columns per workgroup of k_col / rows of k_rowT, k_blue_row: powers of two,
+// 128-byte global segments where the LDS budget allows, >= 256 threads.
+template <typename T, int L> __host__ __device__ constexpr int col_lpb()
+{
+    const int base = sizeof(T) == 4 ? 16 : 8;
+    const int occ = 256 / Sched<L>::NT;
+    int v = occ > base ? occ : base;
+    while (v > 1 && ((Sched<L>::S > 1 && (long)L * v * (long)sizeof(Cx<T>) > 128 * 1024) ||
+                     Sched<L>::NT * v > 1024))
+        v /= 2;
+    return v;
+}
+template <typename T, int L> __host__ __device__ constexpr int rowT_lpb()
+{
+    const int base = sizeof(T) == 4 ? 16 : 8;
+    const int occ = 256 / Sched<L>::NT;
+    int v = occ > base ? occ : base;
+    while (v > 1 && ((long)L * (v + 1) * (long)sizeof(Cx<T>) > 150 * 1024 ||
+                     (long)line_lds<T, L, 1>() * v * (long)sizeof(Cx<T>) > 150 * 1024 || Sched<L>::NT * v > 1024))
+        v /= 2;
+    return v;
+}
+template <typename T, int L> __host__ __device__ constexpr int bluerow_lpb()
+{
+    int v = 256 / Sched<L>::NT;
+    if (v < 1) v = 1;
+    while (v > 1 && ((long)line_lds<T, L, 1>() * v * (long)sizeof(Cx<T>) > 150 * 1024 || Sched<L>::NT * v > 1024))
+        v /= 2;
+    return v;
+}
+
+// runtime mirrors of the above (pow2 L only)
+int col_lpb_rt(int prec, int L);
+int rowT_lpb_rt(int prec, int L);
+int bluerow_lpb_rt(int prec, int L);
+int line_lpb_rt(int L);
+
+// ---------------------------------------------------------------------------
+// launchers (nfft_k_*.hip); return hipSuccess or the launch error
+// ---------------------------------------------------------------------------
+hipError_t launch_line(int prec, int L, int dir, const LineArgs &a, hipStream_t s);
+hipError_t launch_blue_line(int prec, int M, const BlueArgs &a, hipStream_t s);
+hipError_t launch_col(int prec, int L, int dir, int mode, const ColArgs &a, hipStream_t s);
+hipError_t launch_rowT(int prec, int L, int dir, const RowTArgs &a, hipStream_t s);
+hipError_t launch_blue_row(int prec, int L, const BlueRowArgs &a, hipStream_t s);
+hipError_t launch_r2c(int prec, int L, const RtArgs &a, hipStream_t s);
+hipError_t launch_c2r(int prec, int L, const RtArgs &a, hipStream_t s);
+hipError_t launch_t23(int prec, int L, int op, const RtArgs &a, hipStream_t s);
+hipError_t launch_t4(int prec, int L, int op, const RtArgs &a, hipStream_t s);
+
+enum EwKind { EW_UNPACK = 0, EW_PACK = 1, EW_TRIG_PRE = 2, EW_TRIG_POST = 3, EW_N1 = 4 };
+hipError_t launch_ew(int prec, int kind, const EwArgs &a, long long nthreads_x, hipStream_t s);
+
+// table generation: out[i] = e^{-2 pi i m(i)/den} in FP64 (out64) and/or
+// rounded to FP32 (out32); mode 0: m = (a*i + c) mod den, 1: m = (i>>4)*(i&15),
+// 2: m = (i*i) mod den.
+hipError_t launch_gen_phase(double2 *out64, float2 *out32, long long count, int mode, long long a,
+                            long long c, long long den, hipStream_t s);
+// Bluestein filter h (length M): conj chirp wrapped (FP64)
+hipError_t launch_blue_filter(double2 *h, const double2 *chirp64, int n, int m, hipStream_t s);
+// H (FP64, natural) -> scaled, optionally permuted [k1][k2] = H[k1 + m1*k2], to FP32/FP64
+hipError_t launch_blue_hprep(const double2 *H, void *out, int prec, int m, int m1, int m2, double scale,
+                             hipStream_t s);
+hipError_t launch_fill_uniform(void *out, long long count, uint64_t seed, uint64_t start, int prec,
+                               hipStream_t s);
+// strided transpose of a batch of [rows x cols] matrices of elements of
+// `esz` bytes (4, 8 or 16): out[b][c][r] = in[b][r][c]
+hipError_t launch_transpose(const void *in, void *out, long long batch, long long rows, long long cols, int esz,
+                            hipStream_t s);
+// elementwise complex ops for the adjoint / loss helpers
+hipError_t launch_scale_real(void *buf, long long count, int prec, double s, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// per-device table cache (FP64-generated twiddles, shared by all plans)
+// ---------------------------------------------------------------------------
+struct StageTw {
+    const void *ptr;  // Sched<L>::TWN entries of Cx<T>
+};
+const void *stage_table(int prec, int L);                         // nullptr on failure
+bool tw2_table(long long n, Tw2 *out);                            // FP64 two-level e^{-2 pi i e/n}
+const void *phase_table(int prec, long long count, long long a, long long c, long long den);
+const void *chirp_table(int prec, long long n, const double2 **chirp64);
+
+// ---------------------------------------------------------------------------
+// 1D complex engine over contiguous lines
+// ---------------------------------------------------------------------------
+enum C2CAlgo { ALG_LINE = 0, ALG_FOUR = 1, ALG_BLUE1 = 2, ALG_BLUE3 = 3, ALG_N1 = 4 };
+
+struct C2C {
+    int n = 0;
+    int prec = F32;
+    int algo = ALG_LINE;
+    int L = 0;           // ALG_LINE: n; ALG_BLUE1: m
+    int n1 = 0, n2 = 0;  // ALG_FOUR: columns FFT length n1 over n2 columns, then rows of n2
+    int m = 0, m1 = 0, m2 = 0;
+    const void *tw_a = nullptr, *tw_b = nullptr;  // stage tables (L or n1/m1, n2/m2)
+    Tw2 tw2{};
+    const void *chirp = nullptr;
+    void *H = nullptr;   // owned: Bluestein filter spectrum (scaled 1/m)
+    long long scratch_per_line = 0;  // elements of Cx per line (ALG_FOUR: n, ALG_BLUE3: m)
+
+    // Set up tables for length n; returns false with the error set.
+    bool init(int n, int prec);
+    void release();
+    std::string describe() const;
+    // dir: -1 forward, +1 inverse (unnormalised); scale applied to the output.
+    // scratch: scratch_lines * scratch_per_line elements (may be null when
+    // scratch_per_line == 0).
+    hipError_t run(const void *in, void *out, long long nlines, long long in_dist, long long out_dist, int dir,
+                   double scale, void *scratch, long long scratch_lines, hipStream_t s) const;
+};
+
+// strided complex transform along one axis of a row-major array:
+// element (a, i, b) at a*n*B + i*B + b, in place on `buf`.
+hipError_t c2c_axis(const C2C &e, void *buf, long long A, long long B, int dir, double scale, void *scratch,
+                    long long scratch_elems, hipStream_t s);
+
+}  // namespace nfft

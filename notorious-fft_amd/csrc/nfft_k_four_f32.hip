@@ -1,0 +1,8 @@
+// Instantiates the column, transposed-row and Bluestein-row passes for float.
+#include "nfft_launch_impl.h"
+
+namespace nfft {
+template hipError_t col_impl<float>(int, int, int, const ColArgs &, hipStream_t);
+template hipError_t rowT_impl<float>(int, int, const RowTArgs &, hipStream_t);
+template hipError_t blue_row_impl<float>(int, const BlueRowArgs &, hipStream_t);
+}  // namespace nfft

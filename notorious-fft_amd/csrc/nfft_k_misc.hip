@@ -1,0 +1,272 @@
+// nfft_k_misc.hip -- FP64 table generation on the device, Bluestein filter
+// preparation, elementwise real/trig passes, tiled transposes, the synthetic
+// input generator and the precision dispatch of every launcher.
+#include "nfft_launch_impl.h"
+
+namespace nfft {
+
+// ---------------------------------------------------------------------------
+// precision dispatch
+// ---------------------------------------------------------------------------
+hipError_t launch_line(int prec, int L, int dir, const LineArgs &a, hipStream_t s)
+{
+    return prec == F32 ? line_impl<float>(L, dir, a, s) : line_impl<double>(L, dir, a, s);
+}
+hipError_t launch_blue_line(int prec, int M, const BlueArgs &a, hipStream_t s)
+{
+    return prec == F32 ? blue_line_impl<float>(M, a, s) : blue_line_impl<double>(M, a, s);
+}
+hipError_t launch_col(int prec, int L, int dir, int mode, const ColArgs &a, hipStream_t s)
+{
+    return prec == F32 ? col_impl<float>(L, dir, mode, a, s) : col_impl<double>(L, dir, mode, a, s);
+}
+hipError_t launch_rowT(int prec, int L, int dir, const RowTArgs &a, hipStream_t s)
+{
+    return prec == F32 ? rowT_impl<float>(L, dir, a, s) : rowT_impl<double>(L, dir, a, s);
+}
+hipError_t launch_blue_row(int prec, int L, const BlueRowArgs &a, hipStream_t s)
+{
+    return prec == F32 ? blue_row_impl<float>(L, a, s) : blue_row_impl<double>(L, a, s);
+}
+hipError_t launch_r2c(int prec, int L, const RtArgs &a, hipStream_t s)
+{
+    return prec == F32 ? r2c_impl<float>(L, false, a, s) : r2c_impl<double>(L, false, a, s);
+}
+hipError_t launch_c2r(int prec, int L, const RtArgs &a, hipStream_t s)
+{
+    return prec == F32 ? r2c_impl<float>(L, true, a, s) : r2c_impl<double>(L, true, a, s);
+}
+hipError_t launch_t23(int prec, int L, int op, const RtArgs &a, hipStream_t s)
+{
+    return prec == F32 ? t23_impl<float>(L, op, a, s) : t23_impl<double>(L, op, a, s);
+}
+hipError_t launch_t4(int prec, int L, int op, const RtArgs &a, hipStream_t s)
+{
+    return prec == F32 ? t4_impl<float>(L, op, a, s) : t4_impl<double>(L, op, a, s);
+}
+
+template <int L> static int col_lpb_of(int prec) { return prec == F32 ? col_lpb<float, L>() : col_lpb<double, L>(); }
+template <int L> static int rowT_lpb_of(int prec) { return prec == F32 ? rowT_lpb<float, L>() : rowT_lpb<double, L>(); }
+template <int L> static int bluerow_lpb_of(int prec)
+{
+    return prec == F32 ? bluerow_lpb<float, L>() : bluerow_lpb<double, L>();
+}
+int col_lpb_rt(int prec, int L)
+{
+    switch (L) {
+#define NFFT_CASE(LL) case LL: return col_lpb_of<LL>(prec);
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return 1;
+}
+int rowT_lpb_rt(int prec, int L)
+{
+    switch (L) {
+#define NFFT_CASE(LL) case LL: return rowT_lpb_of<LL>(prec);
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return 1;
+}
+int bluerow_lpb_rt(int prec, int L)
+{
+    switch (L) {
+#define NFFT_CASE(LL) case LL: return bluerow_lpb_of<LL>(prec);
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return 1;
+}
+int line_lpb_rt(int L)
+{
+    switch (L) {
+#define NFFT_CASE(LL) case LL: return line_lpb<LL>();
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return 1;
+}
+
+// ---------------------------------------------------------------------------
+// elementwise real / trig passes
+// ---------------------------------------------------------------------------
+hipError_t launch_ew(int prec, int kind, const EwArgs &a, long long nx, hipStream_t s)
+{
+    unsigned gy, gz;
+    grid_trans(a.nlines, gy, gz);
+    dim3 b(256), g((unsigned)((nx + 255) / 256), gy, gz);
+    if (kind == EW_N1) g = dim3((unsigned)((a.nlines + 255) / 256));
+#define NFFT_EW(K)                                                                                            \
+    if (prec == F32) hipLaunchKernelGGL((K<float>), g, b, 0, s, a);                                           \
+    else hipLaunchKernelGGL((K<double>), g, b, 0, s, a);
+    switch (kind) {
+    case EW_UNPACK: NFFT_EW(k_ew_unpack) break;
+    case EW_PACK: NFFT_EW(k_ew_pack) break;
+    case EW_TRIG_PRE: NFFT_EW(k_ew_trig_pre) break;
+    case EW_TRIG_POST: NFFT_EW(k_ew_trig_post) break;
+    case EW_N1: NFFT_EW(k_ew_n1) break;
+    default: return hipErrorInvalidValue;
+    }
+#undef NFFT_EW
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// twiddle / chirp tables: e^{-2 pi i m/den}, evaluated in FP64 from the exact
+// integer phase m (reduced mod den), so the argument of sincospi is the
+// correctly rounded rational 2m/den in (-1, 1].
+// ---------------------------------------------------------------------------
+__global__ void k_gen_phase(double2 *o64, float2 *o32, long long count, int mode, long long a, long long c,
+                            long long den)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const unsigned long long ud = (unsigned long long)den;
+    unsigned long long m;
+    if (mode == 0) m = ((unsigned long long)a * (unsigned long long)i + (unsigned long long)c) % ud;
+    else if (mode == 1) m = ((unsigned long long)(i >> 4) * (unsigned long long)(i & 15)) % ud;
+    else m = ((unsigned long long)i * (unsigned long long)i) % ud;
+    double x = 2.0 * (double)m / (double)den;  // in [0, 2)
+    if (x > 1.0) x -= 2.0;                     // in (-1, 1]
+    double sv, cv;
+    sincospi(x, &sv, &cv);
+    if (o64) o64[i] = make_double2(cv, -sv);
+    if (o32) o32[i] = make_float2((float)cv, (float)-sv);
+}
+
+hipError_t launch_gen_phase(double2 *out64, float2 *out32, long long count, int mode, long long a, long long c,
+                            long long den, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gen_phase, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, out64, out32, count,
+                       mode, a, c, den);
+    return hipGetLastError();
+}
+
+// h[m] = conj(a[m]) for m < n, conj(a[M-m]) for M-m < n, else 0 (wrapped
+// symmetric filter of the chirp a[k] = e^{-i pi k^2/n}).
+__global__ void k_blue_filter(double2 *h, const double2 *a, int n, int M)
+{
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= M) return;
+    double2 v = make_double2(0.0, 0.0);
+    if (m < n) v = make_double2(a[m].x, -a[m].y);
+    else if (M - m < n) v = make_double2(a[M - m].x, -a[M - m].y);
+    h[m] = v;
+}
+
+hipError_t launch_blue_filter(double2 *h, const double2 *chirp64, int n, int m, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_blue_filter, dim3((m + 255) / 256), dim3(256), 0, s, h, chirp64, n, m);
+    return hipGetLastError();
+}
+
+// out = scale * H, natural (m1 == 0) or [k1][k2] = H[k1 + m1 k2] order.
+__global__ void k_blue_hprep(const double2 *H, void *out, int prec, int m, int m1, int m2, double scale)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= m) return;
+    int src = idx;
+    if (m1 > 0) {
+        const int k1 = idx / m2, k2 = idx % m2;
+        src = k1 + m1 * k2;
+    }
+    const double2 v = make_double2(H[src].x * scale, H[src].y * scale);
+    if (prec == F32) ((float2 *)out)[idx] = make_float2((float)v.x, (float)v.y);
+    else ((double2 *)out)[idx] = v;
+}
+
+hipError_t launch_blue_hprep(const double2 *H, void *out, int prec, int m, int m1, int m2, double scale,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_blue_hprep, dim3((m + 255) / 256), dim3(256), 0, s, H, out, prec, m, m1, m2, scale);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// synthetic input: counter-based SplitMix64, uniform [-1, 1)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sm64_uniform(unsigned long long seed, unsigned long long idx)
+{
+    unsigned long long z = ((seed << 40) | idx) + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z = z ^ (z >> 31);
+    return 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+}
+
+__global__ void k_fill_uniform(void *out, long long count, unsigned long long seed, unsigned long long start,
+                               int prec)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const double v = sm64_uniform(seed, start + (unsigned long long)i);
+    if (prec == F32) ((float *)out)[i] = (float)v;
+    else ((double *)out)[i] = v;
+}
+
+hipError_t launch_fill_uniform(void *out, long long count, uint64_t seed, uint64_t start, int prec, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    long long blocks = (count + 255) / 256;
+    // one launch per 2^31 blocks is far beyond any buffer here; keep it simple
+    hipLaunchKernelGGL(k_fill_uniform, dim3((unsigned)blocks), dim3(256), 0, s, out, count, seed, start, prec);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// batched transpose through a 32 x 33 LDS tile: out[b][c][r] = in[b][r][c]
+// ---------------------------------------------------------------------------
+template <typename E> __global__ void k_transpose(const E *in, E *out, long long rows, long long cols)
+{
+    __shared__ E tile[32][33];
+    const long long b = blockIdx.z;
+    const long long r0 = (long long)blockIdx.y * 32, c0 = (long long)blockIdx.x * 32;
+    const E *src = in + b * rows * cols;
+    E *dst = out + b * rows * cols;
+    const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const long long r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) tile[k][tx] = src[r * cols + c];
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const long long c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) dst[c * rows + r] = tile[tx][k];
+    }
+}
+
+hipError_t launch_transpose(const void *in, void *out, long long batch, long long rows, long long cols, int esz,
+                            hipStream_t s)
+{
+    if (batch <= 0 || rows <= 0 || cols <= 0) return hipSuccess;
+    if (batch > 65535 || (cols + 31) / 32 > 0x7fffffffLL || (rows + 31) / 32 > 65535) return hipErrorInvalidValue;
+    dim3 g((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32), (unsigned)batch), b(32, 8);
+    switch (esz) {
+    case 4: hipLaunchKernelGGL((k_transpose<float>), g, b, 0, s, (const float *)in, (float *)out, rows, cols); break;
+    case 8: hipLaunchKernelGGL((k_transpose<float2>), g, b, 0, s, (const float2 *)in, (float2 *)out, rows, cols); break;
+    case 16:
+        hipLaunchKernelGGL((k_transpose<double2>), g, b, 0, s, (const double2 *)in, (double2 *)out, rows, cols);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+__global__ void k_scale_real(void *buf, long long count, int prec, double sc)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    if (prec == F32) ((float *)buf)[i] *= (float)sc;
+    else ((double *)buf)[i] *= sc;
+}
+
+hipError_t launch_scale_real(void *buf, long long count, int prec, double sc, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scale_real, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, buf, count, prec, sc);
+    return hipGetLastError();
+}
+
+}  // namespace nfft

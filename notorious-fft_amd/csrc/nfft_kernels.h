@@ -1,0 +1,404 @@
+// nfft_kernels.h -- kernel templates for the CDNA4 FFT engine.
+//
+// Families (each instantiated per precision and line length in the .hip files):
+//   k_line      single-pass batched C2C: one L-point line per L/16 threads,
+//               LPB lines per workgroup, the whole line resident in VGPRs/LDS.
+//   k_col       column pass: LPB adjacent columns of stride S (coalesced across
+//               columns), used for the first four-step pass (with the
+//               inter-pass twiddle), N-D axes and the Bluestein passes.
+//   k_rowT      row pass with transposed store: second four-step pass.
+//   k_blue_line single-kernel Bluestein for M <= Lmax: chirp, FFT_M, x H,
+//               IFFT_M, chirp, all in one workgroup.
+//   k_blue_row  middle pass of the three-pass Bluestein (FFT_M2, x H, IFFT_M2).
+//
+// Replaces the reference's per-call CPU loops: split-radix DIF
+// (src/05_algorithms.h:358-736), Bluestein (src/05_algorithms.h:742-896) and
+// the strided N-D passes (src/07_legacy.h:81-139).
+#pragma once
+#include "nfft_device.h"
+
+namespace nfft {
+
+template <typename T> __device__ __forceinline__ Cx<T> czero() { return cmk<Cx<T>>((T)0, (T)0); }
+
+// Coalesced copy of a tile of rows (row r at src + r*dist, `len` valid points,
+// rows >= nrows or points >= len read as zero) into the line-major LDS
+// layout of lines of length L.
+template <typename T, int L, int LPB, int NTH>
+__device__ __forceinline__ void stage_in_rows(Cx<T> *lds, const Cx<T> *src, long long dist, int len, int nrows)
+{
+    constexpr int PADL = lpadlen<T, L>();
+    for (int idx = threadIdx.x; idx < LPB * L; idx += NTH) {
+        const int b = idx / L, p = idx & (L - 1);
+        Cx<T> v = czero<T>();
+        if (b < nrows && p < len) v = src[b * dist + p];
+        lds[b * PADL + lpad<T>(p)] = v;
+    }
+}
+
+template <typename T, int L, int LPB, int NTH>
+__device__ __forceinline__ void stage_out_rows(const Cx<T> *lds, Cx<T> *dst, long long dist, int len, int nrows)
+{
+    constexpr int PADL = lpadlen<T, L>();
+    for (int idx = threadIdx.x; idx < LPB * L; idx += NTH) {
+        const int b = idx / L, p = idx & (L - 1);
+        if (b < nrows && p < len) dst[b * dist + p] = lds[b * PADL + lpad<T>(p)];
+    }
+}
+
+// LDS elements needed by a line-major kernel of LPB lines.
+template <typename T, int L, int LPB> __host__ __device__ constexpr int line_lds()
+{
+    return LPB * lpadlen<T, L>();
+}
+
+// Lines per workgroup for the single-pass kernels: ~256 threads, >= 1 line.
+template <int L> __host__ __device__ constexpr int line_lpb()
+{
+    return Sched<L>::NT >= 256 ? 1 : 256 / Sched<L>::NT;
+}
+
+// ---------------------------------------------------------------------------
+// Load / store of a tile of contiguous lines into reg[] (line-major mapping).
+// Direct global access when a line spans >= 64 threads (a full wavefront
+// reads 64 consecutive points), LDS-staged coalesced copy otherwise.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int LPB>
+__device__ __forceinline__ void tile_load(Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds, const Cx<T> *src, long long dist,
+                                          int len, int nrows, int b, int j)
+{
+    using S = Sched<L>;
+    constexpr int NTH = S::NT * LPB;
+    if constexpr (S::NT >= 64) {
+        const Cx<T> *s = src + b * dist;
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const int p = j + S::NT * q;
+            reg[q] = (b < nrows && p < len) ? s[p] : czero<T>();
+        }
+    } else {
+        stage_in_rows<T, L, LPB, NTH>(lds, src, dist, len, nrows);
+        __syncthreads();
+        const int base = b * lpadlen<T, L>();
+#pragma unroll
+        for (int q = 0; q < S::R; q++) reg[q] = lds[base + lpad<T>(j + S::NT * q)];
+        __syncthreads();
+    }
+}
+
+template <typename T, int L, int LPB>
+__device__ __forceinline__ void tile_store(const Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds, Cx<T> *dst, long long dist,
+                                           int len, int nrows, int b, int j)
+{
+    using S = Sched<L>;
+    constexpr int NTH = S::NT * LPB;
+    if constexpr (S::NT >= 64) {
+        if (b < nrows) {
+            Cx<T> *d = dst + b * dist;
+#pragma unroll
+            for (int q = 0; q < S::R; q++) {
+                const int p = j + S::NT * q;
+                if (p < len) d[p] = reg[q];
+            }
+        }
+    } else {
+        const int base = b * lpadlen<T, L>();
+#pragma unroll
+        for (int q = 0; q < S::R; q++) lds[base + lpad<T>(j + S::NT * q)] = reg[q];
+        __syncthreads();
+        stage_out_rows<T, L, LPB, NTH>(lds, dst, dist, len, nrows);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_line: batched contiguous C2C lines (in-place allowed).
+// ---------------------------------------------------------------------------
+struct LineArgs {
+    const void *in;
+    void *out;
+    long long nlines;
+    long long in_dist, out_dist;  // elements between consecutive lines
+    const void *tw;               // stage twiddles for L
+    double scale;
+};
+
+template <typename T, int L, int DIR>
+__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_line(LineArgs a)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    constexpr int LPB = line_lpb<L>();
+    constexpr int NLDS = (S::NT >= 64 && S::S == 1) ? 1 : line_lds<T, L, LPB>();
+    __shared__ C lds[NLDS];
+    const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
+    const long long line0 = (long long)blockIdx.x * LPB;
+    const long long rem = a.nlines - line0;
+    const int nrows = rem < LPB ? (int)rem : LPB;
+    const C *src = (const C *)a.in + line0 * a.in_dist;
+    C *dst = (C *)a.out + line0 * a.out_dist;
+    C reg[S::R];
+    tile_load<T, L, LPB>(reg, lds, src, a.in_dist, L, nrows, b, j);
+    LayLine<T, L> lay{b * lpadlen<T, L>()};
+    line_fft<T, L, DIR>(reg, lds, lay, j, (const C *)a.tw);
+    const T sc = (T)a.scale;
+    if (sc != (T)1) {
+#pragma unroll
+        for (int q = 0; q < S::R; q++) reg[q] = cscale(reg[q], sc);
+    }
+    tile_store<T, L, LPB>(reg, lds, dst, a.out_dist, L, nrows, b, j);
+}
+
+// ---------------------------------------------------------------------------
+// k_col: LPB adjacent columns per workgroup; element (t, c, p) lives at
+// base + t*dist + c*cstride + p*S with cstride = 1 for coalescing.
+// MODE selects the prologue/epilogue:
+//   COL_PLAIN     in -> FFT -> out
+//   COL_TWIDDLE   in -> FFT -> * w_N^{DIR' c p} -> out   (four-step pass 1)
+//   COL_BLUE_IN   x[n] * chirp[n] (n < Nv, else 0) -> FFT -> * w_M^{c p} -> out
+//   COL_BLUE_OUT  in -> IFFT -> y[n] = conj?(v * chirp[n]) * scale, n < Nv
+// ---------------------------------------------------------------------------
+enum ColMode { COL_PLAIN = 0, COL_TWIDDLE = 1, COL_BLUE_IN = 2, COL_BLUE_OUT = 3 };
+
+struct ColArgs {
+    const void *in;
+    void *out;
+    int ncols;                 // columns per transform
+    long long S;               // point stride (elements)
+    long long in_dist, out_dist;  // per-transform distance
+    long long ntrans;          // transforms (grid.y * gridDim.z covers them)
+    const void *tw;            // stage twiddles for L
+    Tw2 tw2;                   // inter-pass twiddle (TWIDDLE / BLUE_IN)
+    const void *chirp;         // Bluestein chirp (length Nv)
+    long long nv;              // Bluestein valid length
+    int conj;                  // Bluestein: conjugate input / output
+    double scale;
+};
+
+template <int L, int LPB> __host__ __device__ constexpr int col_threads() { return Sched<L>::NT * LPB; }
+
+template <typename T, int L, int LPB, int DIR, int MODE>
+__global__ void __launch_bounds__((col_threads<L, LPB>())) k_col(ColArgs a)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    constexpr int NLDS = S::S > 1 ? L * LPB : 1;
+    __shared__ C lds[NLDS];
+    const int b = threadIdx.x % LPB, j = threadIdx.x / LPB;
+    const int c = blockIdx.x * LPB + b;
+    const long long t = (long long)blockIdx.y + (long long)blockIdx.z * gridDim.y;
+    const bool valid = c < a.ncols && t < a.ntrans;
+    C reg[S::R];
+    if constexpr (MODE == COL_BLUE_IN) {
+        const C *src = (const C *)a.in + t * a.in_dist;
+        const C *ch = (const C *)a.chirp;
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const long long n = (long long)(j + S::NT * q) * a.S + c;
+            C v = czero<T>();
+            if (valid && n < a.nv) {
+                v = src[n];
+                if (a.conj) v = cconj(v);
+                v = cmul(v, ch[n]);
+            }
+            reg[q] = v;
+        }
+    } else {
+        const C *src = (const C *)a.in + t * a.in_dist + c;
+#pragma unroll
+        for (int q = 0; q < S::R; q++) reg[q] = valid ? src[(long long)(j + S::NT * q) * a.S] : czero<T>();
+    }
+    LayCol<LPB> lay{b};
+    line_fft<T, L, DIR>(reg, lds, lay, j, (const C *)a.tw);
+    if constexpr (MODE == COL_TWIDDLE || MODE == COL_BLUE_IN) {
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const uint32_t e = (uint32_t)c * (uint32_t)(j + S::NT * q);
+            reg[q] = ctw<DIR>(reg[q], to_cx<T>(tw2_get(a.tw2, e)));
+        }
+    }
+    if (!valid) return;
+    if constexpr (MODE == COL_BLUE_OUT) {
+        C *dst = (C *)a.out + t * a.out_dist;
+        const C *ch = (const C *)a.chirp;
+        const T sc = (T)a.scale;
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const long long n = (long long)(j + S::NT * q) * a.S + c;
+            if (n < a.nv) {
+                C v = cmul(reg[q], ch[n]);
+                if (a.conj) v = cconj(v);
+                dst[n] = cscale(v, sc);
+            }
+        }
+    } else {
+        C *dst = (C *)a.out + t * a.out_dist + c;
+        const T sc = (T)a.scale;
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            C v = reg[q];
+            if (sc != (T)1) v = cscale(v, sc);
+            dst[(long long)(j + S::NT * q) * a.S] = v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_rowT: second four-step pass.  Row r (< nrows) of transform t holds L
+// contiguous points; output point k of row r goes to r + nrows*k (natural
+// order of the full transform), written through an LDS transpose so that
+// LPB consecutive rows form one contiguous store segment.
+// ---------------------------------------------------------------------------
+struct RowTArgs {
+    const void *in;
+    void *out;
+    int nrows;
+    long long in_dist, out_dist;  // per-transform distances
+    long long ntrans;
+    const void *tw;
+    double scale;
+};
+
+template <typename T, int L, int LPB> __host__ __device__ constexpr int rowT_lds()
+{
+    return (L * (LPB + 1)) > line_lds<T, L, LPB>() ? L * (LPB + 1) : line_lds<T, L, LPB>();
+}
+
+template <typename T, int L, int LPB, int DIR>
+__global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT(RowTArgs a)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    constexpr int NTH = S::NT * LPB;
+    __shared__ C lds[rowT_lds<T, L, LPB>()];
+    const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
+    const int r0 = blockIdx.x * LPB;
+    const long long t = (long long)blockIdx.y + (long long)blockIdx.z * gridDim.y;
+    if (t >= a.ntrans) return;  // uniform per workgroup
+    const int nrows = (a.nrows - r0) < LPB ? (a.nrows - r0) : LPB;
+    const C *src = (const C *)a.in + t * a.in_dist + (long long)r0 * L;
+    C reg[S::R];
+    tile_load<T, L, LPB>(reg, lds, src, L, L, nrows, b, j);
+    LayLine<T, L> lay{b * lpadlen<T, L>()};
+    line_fft<T, L, DIR>(reg, lds, lay, j, (const C *)a.tw);
+    const T sc = (T)a.scale;
+#pragma unroll
+    for (int q = 0; q < S::R; q++) {
+        C v = reg[q];
+        if (sc != (T)1) v = cscale(v, sc);
+        lds[(j + S::NT * q) * (LPB + 1) + b] = v;
+    }
+    __syncthreads();
+    C *dst = (C *)a.out + t * a.out_dist + r0;
+    for (int idx = threadIdx.x; idx < LPB * L; idx += NTH) {
+        const int bb = idx % LPB, k = idx / LPB;
+        if (bb < nrows) dst[(long long)k * a.nrows + bb] = lds[k * (LPB + 1) + bb];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_blue_line: Bluestein for M <= Lmax in one workgroup per line group.
+// y[k] = conj?( a[k] * IFFT_M( FFT_M(pad(a * conj?(x))) * Hs )[k] ) * scale
+// with a = e^{-i pi k^2/N} and Hs = FFT_M(h)/M (h the wrapped conj chirp).
+// ---------------------------------------------------------------------------
+struct BlueArgs {
+    const void *in;
+    void *out;
+    long long nlines;
+    long long in_dist, out_dist;  // line distances
+    int n;                 // transform length N
+    const void *tw;        // stage twiddles for M
+    const void *chirp;     // N entries
+    const void *H;         // M entries (natural order), includes 1/M
+    int conj;
+    double scale;
+};
+
+template <typename T, int M>
+__global__ void __launch_bounds__(Sched<M>::NT *line_lpb<M>()) k_blue_line(BlueArgs a)
+{
+    using S = Sched<M>;
+    using C = Cx<T>;
+    constexpr int LPB = line_lpb<M>();
+    __shared__ C lds[line_lds<T, M, LPB>()];
+    const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
+    const long long line0 = (long long)blockIdx.x * LPB;
+    const long long rem = a.nlines - line0;
+    const int nrows = rem < LPB ? (int)rem : LPB;
+    const C *src = (const C *)a.in + line0 * a.in_dist;
+    C *dst = (C *)a.out + line0 * a.out_dist;
+    const C *ch = (const C *)a.chirp;
+    const C *H = (const C *)a.H;
+    C reg[S::R];
+    tile_load<T, M, LPB>(reg, lds, src, a.in_dist, a.n, nrows, b, j);
+#pragma unroll
+    for (int q = 0; q < S::R; q++) {
+        const int p = j + S::NT * q;
+        if (p < a.n) {
+            C v = reg[q];
+            if (a.conj) v = cconj(v);
+            reg[q] = cmul(v, ch[p]);
+        }
+    }
+    LayLine<T, M> lay{b * lpadlen<T, M>()};
+    line_fft<T, M, -1>(reg, lds, lay, j, (const C *)a.tw);
+#pragma unroll
+    for (int q = 0; q < S::R; q++) reg[q] = cmul(reg[q], H[j + S::NT * q]);
+    line_fft<T, M, 1>(reg, lds, lay, j, (const C *)a.tw);
+    const T sc = (T)a.scale;
+#pragma unroll
+    for (int q = 0; q < S::R; q++) {
+        const int p = j + S::NT * q;
+        C v = p < a.n ? cmul(reg[q], ch[p]) : czero<T>();
+        if (a.conj) v = cconj(v);
+        reg[q] = cscale(v, sc);
+    }
+    if constexpr (S::NT < 64) __syncthreads();  // staged store reuses LDS
+    tile_store<T, M, LPB>(reg, lds, dst, a.out_dist, a.n, nrows, b, j);
+}
+
+// ---------------------------------------------------------------------------
+// k_blue_row: middle pass of the three-pass Bluestein on an M = M1 x M2
+// buffer (row k1 holds M2 contiguous points): FFT over n2, x Hp[k1][k2],
+// inverse FFT over k2, x w_M^{+n2 k1}; in place.
+// ---------------------------------------------------------------------------
+struct BlueRowArgs {
+    void *buf;
+    int nrows;            // M1
+    long long dist;       // M per transform
+    long long ntrans;
+    const void *tw;       // stage twiddles for M2
+    const void *Hp;       // M entries, [k1][k2] order, includes 1/M
+    Tw2 tw2;              // w_M
+};
+
+template <typename T, int L, int LPB>
+__global__ void __launch_bounds__(Sched<L>::NT *LPB) k_blue_row(BlueRowArgs a)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    __shared__ C lds[line_lds<T, L, LPB>()];
+    const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
+    const int r0 = blockIdx.x * LPB;
+    const long long t = (long long)blockIdx.y + (long long)blockIdx.z * gridDim.y;
+    if (t >= a.ntrans) return;
+    const int nrows = (a.nrows - r0) < LPB ? (a.nrows - r0) : LPB;
+    C *buf = (C *)a.buf + t * a.dist + (long long)r0 * L;
+    C reg[S::R];
+    tile_load<T, L, LPB>(reg, lds, buf, L, L, nrows, b, j);
+    LayLine<T, L> lay{b * lpadlen<T, L>()};
+    line_fft<T, L, -1>(reg, lds, lay, j, (const C *)a.tw);
+    const int k1 = r0 + (b < nrows ? b : 0);
+    const C *Hp = (const C *)a.Hp + (long long)k1 * L;
+#pragma unroll
+    for (int q = 0; q < S::R; q++) reg[q] = cmul(reg[q], Hp[j + S::NT * q]);
+    line_fft<T, L, 1>(reg, lds, lay, j, (const C *)a.tw);
+#pragma unroll
+    for (int q = 0; q < S::R; q++) {
+        const uint32_t e = (uint32_t)k1 * (uint32_t)(j + S::NT * q);
+        reg[q] = cmulc(reg[q], to_cx<T>(tw2_get(a.tw2, e)));
+    }
+    if constexpr (S::NT < 64) __syncthreads();
+    tile_store<T, L, LPB>(reg, lds, buf, L, L, nrows, b, j);
+}
+
+}  // namespace nfft

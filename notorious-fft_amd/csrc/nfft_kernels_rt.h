@@ -1,0 +1,487 @@
+// nfft_kernels_rt.h -- real-input and trigonometric transform kernels.
+//
+// All of them run an L-point complex line (L = N/2) with the pack/unpack and
+// pre/post twiddles fused into the same workgroup, the line resident in LDS:
+//   k_r2c_line   realdft    (src/07_legacy.h:162-237): pack N reals as N/2
+//                complex, FFT, unpack N/2+1 Hermitian bins.
+//   k_c2r_line   invrealdft (src/07_legacy.h:239-293): pack, inverse FFT,
+//                unnormalised (invrealdft(realdft(x)) = N x).
+//   k_t23_line   dct2/dst2  (src/07_legacy.h:757-866,867-949): even/odd
+//                reorder on load, realdft, post-twiddle e^{-i pi k/(2N)}.
+//                dct3/dst3  (src/07_legacy.h:951-1066): pre-twiddle,
+//                invrealdft, un-reorder on store.
+//   k_t4_line    dct4/dst4  (src/07_legacy.h:1091-1137,1298-1334): premul
+//                e^{-i pi n/N}, N/2-point FFT, postmul e^{-i pi (2n+1)/(4N)}.
+// For N/2 above the single-pass limit the same arithmetic runs as separate
+// elementwise kernels around the multi-pass complex FFT (k_rt_* below).
+#pragma once
+#include "nfft_kernels.h"
+
+namespace nfft {
+
+enum TrigOp { T_DCT2 = 0, T_DST2 = 1, T_DCT3 = 2, T_DST3 = 3, T_DCT4 = 4, T_DST4 = 5 };
+
+struct RtArgs {
+    const void *in;
+    void *out;
+    long long nlines;
+    long long in_dist, out_dist;  // in units of the element type on each side
+    const void *tw;               // stage twiddles for L
+    const void *t1;               // realdft: e^{-2 pi i k/N} (k <= L); t23: e^{-i pi k/(2N)} (k < L);
+                                  // t4: premul e^{-i pi n/N} (n < L)
+    const void *t2;               // t23: realdft twiddles; t4: postmul (n < N)
+    double scale;
+    double w0;                    // weight of the DC term (vjp weightings)
+    int half;                     // c2r: weight 1/2 on bins 0 < k < N/2 (realdft vjp)
+};
+
+// real view of a padded complex LDS line: real index i -> slot i>>1, comp i&1
+template <typename T, int L> __device__ __forceinline__ T lds_real(const Cx<T> *line, int i)
+{
+    const Cx<T> v = line[lpad<T>(i >> 1)];
+    return (i & 1) ? v.y : v.x;
+}
+
+// X[k] of the length-2L real DFT from the L-point complex DFT Z of the
+// packed sequence: E = (Zk + conj Zm)/2, O = -i (Zk - conj Zm)/2,
+// X[k] = E + w O (w = e^{-2 pi i k/(2L)}), X[L] from k = 0 as E - O.
+template <typename C> __device__ __forceinline__ void unpack_pair(C zk, C zm, C w, C &xk, C &xl)
+{
+    using T = decltype(C::x);
+    const T h = (T)0.5;
+    C e = cmk<C>(h * (zk.x + zm.x), h * (zk.y - zm.y));
+    C o = cmk<C>(h * (zk.y + zm.y), -h * (zk.x - zm.x));
+    xk = cadd(e, cmul(w, o));
+    xl = csub(e, o);
+}
+
+// Z[k] = (Xk + conj Xm) + i conj(w) (Xk - conj Xm), w = e^{-2 pi i k/(2L)}:
+// inverse FFT of Z gives N * (y[2n] + i y[2n+1]).
+template <typename C> __device__ __forceinline__ C pack_pair(C xk, C xm, C w)
+{
+    C s = cmk<C>(xk.x + xm.x, xk.y - xm.y);
+    C d = cmk<C>(xk.x - xm.x, xk.y + xm.y);
+    C t = cmulc(d, w);
+    return cmk<C>(s.x - t.y, s.y + t.x);
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, int L>
+__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_r2c_line(RtArgs a)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    constexpr int LPB = line_lpb<L>();
+    __shared__ C lds[line_lds<T, L, LPB>()];
+    const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
+    const long long line0 = (long long)blockIdx.x * LPB;
+    const long long rem = a.nlines - line0;
+    const int nrows = rem < LPB ? (int)rem : LPB;
+    const C *src = (const C *)((const T *)a.in + line0 * a.in_dist);
+    C reg[S::R];
+    tile_load<T, L, LPB>(reg, lds, src, a.in_dist / 2, L, nrows, b, j);
+    LayLine<T, L> lay{b * lpadlen<T, L>()};
+    line_fft<T, L, -1>(reg, lds, lay, j, (const C *)a.tw);
+    C *line = lds + b * lpadlen<T, L>();
+#pragma unroll
+    for (int q = 0; q < S::R; q++) line[lpad<T>(j + S::NT * q)] = reg[q];
+    __syncthreads();
+    if (b >= nrows) return;
+    C *dst = (C *)a.out + (line0 + b) * a.out_dist;
+    const C *w = (const C *)a.t1;
+    const T sc = (T)a.scale, sk = a.half ? (T)2 * sc : sc;  // half: adjoint weighting 2 on 0 < k < L
+#pragma unroll
+    for (int q = 0; q < S::R; q++) {
+        const int k = j + S::NT * q;
+        const C zk = line[lpad<T>(k)], zm = line[lpad<T>((L - k) & (L - 1))];
+        C xk, xl;
+        unpack_pair(zk, zm, w[k], xk, xl);
+        dst[k] = cscale(xk, k == 0 ? sc : sk);
+        if (k == 0) dst[L] = cscale(xl, sc);
+    }
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_c2r_line(RtArgs a)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    constexpr int LPB = line_lpb<L>();
+    __shared__ C lds[line_lds<T, L, LPB>()];
+    const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
+    const long long line0 = (long long)blockIdx.x * LPB;
+    const long long rem = a.nlines - line0;
+    const int nrows = rem < LPB ? (int)rem : LPB;
+    const C *src = (const C *)a.in + (line0 + (b < nrows ? b : 0)) * a.in_dist;
+    const C *w = (const C *)a.t1;
+    C reg[S::R];
+    const T hw = a.half ? (T)0.5 : (T)1;
+#pragma unroll
+    for (int q = 0; q < S::R; q++) {
+        const int k = j + S::NT * q;
+        C xk = src[k], xm = src[L - k];
+        if (k == 0) {  // bins 0 and N/2 are real (imaginary parts ignored)
+            xk.y = 0;
+            xm.y = 0;
+        } else {
+            xk = cscale(xk, hw);
+            xm = cscale(xm, hw);
+        }
+        reg[q] = pack_pair(xk, xm, w[k]);
+    }
+    LayLine<T, L> lay{b * lpadlen<T, L>()};
+    line_fft<T, L, 1>(reg, lds, lay, j, (const C *)a.tw);
+    const T sc = (T)a.scale;
+    if (sc != (T)1) {
+#pragma unroll
+        for (int q = 0; q < S::R; q++) reg[q] = cscale(reg[q], sc);
+    }
+    C *dst = (C *)((T *)a.out + line0 * a.out_dist);
+    tile_store<T, L, LPB>(reg, lds, dst, a.out_dist / 2, L, nrows, b, j);
+}
+
+// DCT/DST types 2 and 3 -----------------------------------------------------
+template <typename T, int L, int OP>
+__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs a)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    constexpr int LPB = line_lpb<L>();
+    constexpr int NTH = S::NT * LPB;
+    constexpr int N = 2 * L;
+    __shared__ C lds[line_lds<T, L, LPB>()];
+    const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
+    const long long line0 = (long long)blockIdx.x * LPB;
+    const long long rem = a.nlines - line0;
+    const int nrows = rem < LPB ? (int)rem : LPB;
+    const C *src = (const C *)((const T *)a.in + line0 * a.in_dist);
+    stage_in_rows<T, L, LPB, NTH>(lds, src, a.in_dist / 2, L, nrows);
+    __syncthreads();
+    const C *line = lds + b * lpadlen<T, L>();
+    C reg[S::R];
+    LayLine<T, L> lay{b * lpadlen<T, L>()};
+    const T sq2 = (T)1.41421356237309504880;
+    if constexpr (OP == T_DCT2 || OP == T_DST2) {
+        constexpr T sg = OP == T_DCT2 ? (T)1 : (T)-1;
+        // v[i] = x[2i] (i < L), v[L+i] = sg * x[N-1-2i]; z[m] = (v[2m], v[2m+1])
+        auto xv = [&](int i) -> T {
+            return i < L ? lds_real<T, L>(line, 2 * i) : sg * lds_real<T, L>(line, N - 1 - 2 * (i - L));
+        };
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const int m = j + S::NT * q;
+            reg[q] = cmk<C>(xv(2 * m), xv(2 * m + 1));
+        }
+        __syncthreads();
+        line_fft<T, L, -1>(reg, lds, lay, j, (const C *)a.tw);
+        C *ln = lds + b * lpadlen<T, L>();
+#pragma unroll
+        for (int q = 0; q < S::R; q++) ln[lpad<T>(j + S::NT * q)] = reg[q];
+        __syncthreads();
+        // outputs for k = j + NT q: (y[k], y[N-k]) (DCT2) from X[k]; X[0] and X[L] special
+        const C *rw = (const C *)a.t2;   // e^{-2 pi i k/N}
+        const C *dw = (const C *)a.t1;   // e^{-i pi k/(2N)}
+        T ya[S::R], yb[S::R];
+        T yl = 0;
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const int k = j + S::NT * q;
+            const C zk = ln[lpad<T>(k)], zm = ln[lpad<T>((L - k) & (L - 1))];
+            C xk, xl;
+            unpack_pair(zk, zm, rw[k], xk, xl);
+            if (k == 0) {
+                ya[q] = (T)2 * xk.x * (T)a.w0;
+                yb[q] = 0;
+                yl = sq2 * xl.x;
+            } else {
+                const C p = cmul(xk, dw[k]);
+                ya[q] = (T)2 * p.x;
+                yb[q] = (T)-2 * p.y;
+            }
+        }
+        __syncthreads();
+        // scatter y into the LDS line (real view), then a coalesced store
+        const T sc = (T)a.scale;
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const int k = j + S::NT * q;
+            int ia, ib;
+            if constexpr (OP == T_DCT2) {
+                ia = k;
+                ib = N - k;
+            } else {
+                ia = k == 0 ? N - 1 : k - 1;
+                ib = N - k - 1;
+            }
+            const T va = OP == T_DCT2 ? ya[q] : (k == 0 ? ya[q] : yb[q]);
+            const T vb = OP == T_DCT2 ? yb[q] : ya[q];
+            auto put = [&](int i, T v) {
+                C *slot = ln + lpad<T>(i >> 1);
+                if (i & 1) slot->y = v * sc;
+                else slot->x = v * sc;
+            };
+            put(ia, va);
+            if (k != 0) put(ib, vb);
+            else put(OP == T_DCT2 ? L : L - 1, yl);
+        }
+        __syncthreads();
+        C *dst = (C *)((T *)a.out + line0 * a.out_dist);
+        stage_out_rows<T, L, LPB, NTH>(lds, dst, a.out_dist / 2, L, nrows);
+    } else {
+        // DCT3: X[0] = w0 x0, X[k] = conj((x[k] + i x[N-k]) w_k), X[L] = sqrt2 x[L]
+        // DST3: X[0] = w0 x[N-1], X[k] = conj((x[N-1-k] + i x[k-1]) w_k), X[L] = sqrt2 x[L-1]
+        const C *dw = (const C *)a.t1;
+        const C *rw = (const C *)a.t2;
+        auto X = [&](int k) -> C {
+            if (k == 0) {
+                const T v = OP == T_DCT3 ? lds_real<T, L>(line, 0) : lds_real<T, L>(line, N - 1);
+                return cmk<C>(v * (T)a.w0, (T)0);
+            }
+            if (k == L) {
+                const T v = OP == T_DCT3 ? lds_real<T, L>(line, L) : lds_real<T, L>(line, L - 1);
+                return cmk<C>(sq2 * v, (T)0);
+            }
+            T xr, xi;
+            if constexpr (OP == T_DCT3) {
+                xr = lds_real<T, L>(line, k);
+                xi = lds_real<T, L>(line, N - k);
+            } else {
+                xr = lds_real<T, L>(line, N - 1 - k);
+                xi = lds_real<T, L>(line, k - 1);
+            }
+            return cconj(cmul(cmk<C>(xr, xi), dw[k]));
+        };
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const int k = j + S::NT * q;
+            reg[q] = pack_pair(X(k), X(L - k), rw[k]);
+        }
+        __syncthreads();
+        line_fft<T, L, 1>(reg, lds, lay, j, (const C *)a.tw);
+        C *ln = lds + b * lpadlen<T, L>();
+#pragma unroll
+        for (int q = 0; q < S::R; q++) ln[lpad<T>(j + S::NT * q)] = reg[q];
+        __syncthreads();
+        // temp = real view of ln; y pair m = (temp[m], sg * temp[N-1-m])
+        constexpr T sg = OP == T_DCT3 ? (T)1 : (T)-1;
+        const T sc = (T)a.scale;
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const int m = j + S::NT * q;
+            reg[q] = cmk<C>(lds_real<T, L>(ln, m) * sc, sg * lds_real<T, L>(ln, N - 1 - m) * sc);
+        }
+        __syncthreads();
+        C *dst = (C *)((T *)a.out + line0 * a.out_dist);
+        tile_store<T, L, LPB>(reg, lds, dst, a.out_dist / 2, L, nrows, b, j);
+    }
+}
+
+// DCT/DST type 4 -------------------------------------------------------------
+template <typename T, int L, int OP>
+__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t4_line(RtArgs a)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    constexpr int LPB = line_lpb<L>();
+    constexpr int NTH = S::NT * LPB;
+    constexpr int N = 2 * L;
+    __shared__ C lds[line_lds<T, L, LPB>()];
+    const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
+    const long long line0 = (long long)blockIdx.x * LPB;
+    const long long rem = a.nlines - line0;
+    const int nrows = rem < LPB ? (int)rem : LPB;
+    const C *src = (const C *)((const T *)a.in + line0 * a.in_dist);
+    stage_in_rows<T, L, LPB, NTH>(lds, src, a.in_dist / 2, L, nrows);
+    __syncthreads();
+    const C *line = lds + b * lpadlen<T, L>();
+    const C *pre = (const C *)a.t1;
+    const C *post = (const C *)a.t2;
+    C reg[S::R];
+#pragma unroll
+    for (int q = 0; q < S::R; q++) {
+        const int n = j + S::NT * q;
+        const T xe = lds_real<T, L>(line, 2 * n), xo = lds_real<T, L>(line, N - 1 - 2 * n);
+        if constexpr (OP == T_DCT4) reg[q] = cmul(pre[n], cmk<C>(xe, xo));
+        else reg[q] = cneg(cmul(pre[n], cmk<C>(xe, -xo)));
+    }
+    __syncthreads();
+    LayLine<T, L> lay{b * lpadlen<T, L>()};
+    line_fft<T, L, -1>(reg, lds, lay, j, (const C *)a.tw);
+    C *ln = lds + b * lpadlen<T, L>();
+#pragma unroll
+    for (int q = 0; q < S::R; q++) ln[lpad<T>(j + S::NT * q)] = reg[q];
+    __syncthreads();
+    const T sc = (T)a.scale;
+#pragma unroll
+    for (int q = 0; q < S::R; q++) {
+        const int k = j + S::NT * q;
+        const C tk = ln[lpad<T>(k)], tm = cconj(ln[lpad<T>(L - 1 - k)]);
+        const C p0 = cmul(post[2 * k], tk), p1 = cmul(post[2 * k + 1], tm);
+        if constexpr (OP == T_DCT4) reg[q] = cmk<C>((T)2 * p0.x * sc, (T)2 * p1.x * sc);
+        else reg[q] = cmk<C>((T)2 * p0.y * sc, (T)2 * p1.y * sc);
+    }
+    __syncthreads();
+    C *dst = (C *)((T *)a.out + line0 * a.out_dist);
+    tile_store<T, L, LPB>(reg, lds, dst, a.out_dist / 2, L, nrows, b, j);
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise kernels for the multi-pass (large N) real and trig paths and
+// for N == 1.  One thread per output index; grid.y = line.
+// ---------------------------------------------------------------------------
+struct EwArgs {
+    const void *in;
+    void *out;
+    long long nlines;
+    long long in_dist, out_dist;
+    int L;             // N/2
+    const void *t1;
+    const void *t2;
+    double scale;
+    double w0;
+    int half;
+    int op;
+};
+
+// realdft unpack: Z (L complex) -> X (L+1 complex); thread k < L.
+template <typename T> __global__ void k_ew_unpack(EwArgs a)
+{
+    using C = Cx<T>;
+    const int L = a.L;
+    const long long t = (long long)blockIdx.y + (long long)blockIdx.z * gridDim.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= L || t >= a.nlines) return;
+    const C *z = (const C *)a.in + t * a.in_dist;
+    C *x = (C *)a.out + t * a.out_dist;
+    C xk, xl;
+    unpack_pair(z[k], z[(L - k) & (L - 1)], ((const C *)a.t1)[k], xk, xl);
+    const T sc = (T)a.scale, sk = a.half ? (T)2 * sc : sc;
+    x[k] = cscale(xk, k == 0 ? sc : sk);
+    if (k == 0) x[L] = cscale(xl, sc);
+}
+
+// invrealdft pack: X (L+1 complex) -> Z (L complex); thread k < L.
+template <typename T> __global__ void k_ew_pack(EwArgs a)
+{
+    using C = Cx<T>;
+    const int L = a.L;
+    const long long t = (long long)blockIdx.y + (long long)blockIdx.z * gridDim.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= L || t >= a.nlines) return;
+    const C *x = (const C *)a.in + t * a.in_dist;
+    C *z = (C *)a.out + t * a.out_dist;
+    C xk = x[k], xm = x[L - k];
+    const T hw = a.half ? (T)0.5 : (T)1;
+    if (k == 0) {
+        xk.y = 0;
+        xm.y = 0;
+    } else {
+        xk = cscale(xk, hw);
+        xm = cscale(xm, hw);
+    }
+    z[k] = pack_pair(xk, xm, ((const C *)a.t1)[k]);
+}
+
+// Trig pre-processing into a complex buffer of L (dct2/dst2: packed reorder,
+// dct4/dst4: premul) or L+1 (dct3/dst3: pre-twiddled half spectrum) entries.
+template <typename T> __global__ void k_ew_trig_pre(EwArgs a)
+{
+    using C = Cx<T>;
+    const int L = a.L, N = 2 * L;
+    const long long t = (long long)blockIdx.y + (long long)blockIdx.z * gridDim.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nlines) return;
+    const T *x = (const T *)a.in + t * a.in_dist;
+    C *z = (C *)a.out + t * a.out_dist;
+    const T sq2 = (T)1.41421356237309504880;
+    if (a.op == T_DCT2 || a.op == T_DST2) {
+        if (k >= L) return;
+        const T sg = a.op == T_DCT2 ? (T)1 : (T)-1;
+        auto xv = [&](int i) -> T { return i < L ? x[2 * i] : sg * x[N - 1 - 2 * (i - L)]; };
+        z[k] = cmk<C>(xv(2 * k), xv(2 * k + 1));
+    } else if (a.op == T_DCT3 || a.op == T_DST3) {
+        if (k > L) return;
+        C v;
+        if (k == 0) v = cmk<C>((a.op == T_DCT3 ? x[0] : x[N - 1]) * (T)a.w0, (T)0);
+        else if (k == L) v = cmk<C>(sq2 * (a.op == T_DCT3 ? x[L] : x[L - 1]), (T)0);
+        else {
+            const T xr = a.op == T_DCT3 ? x[k] : x[N - 1 - k];
+            const T xi = a.op == T_DCT3 ? x[N - k] : x[k - 1];
+            v = cconj(cmul(cmk<C>(xr, xi), ((const C *)a.t1)[k]));
+        }
+        z[k] = v;
+    } else {
+        if (k >= L) return;
+        const C pre = ((const C *)a.t1)[k];
+        const T xe = x[2 * k], xo = x[N - 1 - 2 * k];
+        z[k] = a.op == T_DCT4 ? cmul(pre, cmk<C>(xe, xo)) : cneg(cmul(pre, cmk<C>(xe, -xo)));
+    }
+}
+
+// Trig post-processing from a complex buffer to N reals.
+//   dct2/dst2: in = X (L+1 bins of the realdft of the reorder)
+//   dct3/dst3: in = temp (N reals, the invrealdft output)
+//   dct4/dst4: in = t (L complex after the forward FFT)
+template <typename T> __global__ void k_ew_trig_post(EwArgs a)
+{
+    using C = Cx<T>;
+    const int L = a.L, N = 2 * L;
+    const long long t = (long long)blockIdx.y + (long long)blockIdx.z * gridDim.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= L || t >= a.nlines) return;
+    T *y = (T *)a.out + t * a.out_dist;
+    const T sc = (T)a.scale;
+    const T sq2 = (T)1.41421356237309504880;
+    if (a.op == T_DCT2 || a.op == T_DST2) {
+        const C *X = (const C *)a.in + t * a.in_dist;
+        if (k == 0) {
+            const T d = (T)2 * X[0].x * (T)a.w0, l = sq2 * X[L].x;
+            if (a.op == T_DCT2) { y[0] = d * sc; y[L] = l * sc; }
+            else { y[N - 1] = d * sc; y[L - 1] = l * sc; }
+            return;
+        }
+        const C p = cmul(X[k], ((const C *)a.t1)[k]);
+        if (a.op == T_DCT2) { y[k] = (T)2 * p.x * sc; y[N - k] = (T)-2 * p.y * sc; }
+        else { y[k - 1] = (T)-2 * p.y * sc; y[N - k - 1] = (T)2 * p.x * sc; }
+    } else if (a.op == T_DCT3 || a.op == T_DST3) {
+        const T *tmp = (const T *)a.in + t * a.in_dist;
+        const T sg = a.op == T_DCT3 ? (T)1 : (T)-1;
+        y[2 * k] = tmp[k] * sc;
+        y[N - 1 - 2 * k] = sg * tmp[L + k] * sc;
+    } else {
+        const C *tt = (const C *)a.in + t * a.in_dist;
+        const C *post = (const C *)a.t2;
+        const C p0 = cmul(post[2 * k], tt[k]), p1 = cmul(post[2 * k + 1], cconj(tt[L - 1 - k]));
+        if (a.op == T_DCT4) { y[2 * k] = (T)2 * p0.x * sc; y[2 * k + 1] = (T)2 * p1.x * sc; }
+        else { y[2 * k] = (T)2 * p0.y * sc; y[2 * k + 1] = (T)2 * p1.y * sc; }
+    }
+}
+
+// N == 1 for every op: y = f * x (complex or real), one thread per line.
+template <typename T> __global__ void k_ew_n1(EwArgs a)
+{
+    using C = Cx<T>;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= a.nlines) return;
+    const T f = (T)a.scale;
+    switch (a.op) {
+    case 0: {  // complex copy * scale
+        ((C *)a.out)[t * a.out_dist] = cscale(((const C *)a.in)[t * a.in_dist], f);
+        break;
+    }
+    case 1: {  // real -> complex (realdft N=1)
+        ((C *)a.out)[t * a.out_dist] = cmk<C>(((const T *)a.in)[t * a.in_dist] * f, (T)0);
+        break;
+    }
+    case 2: {  // complex -> real (invrealdft N=1)
+        ((T *)a.out)[t * a.out_dist] = ((const C *)a.in)[t * a.in_dist].x * f;
+        break;
+    }
+    default: {  // real -> real
+        ((T *)a.out)[t * a.out_dist] = ((const T *)a.in)[t * a.in_dist] * f;
+        break;
+    }
+    }
+}
+
+}  // namespace nfft

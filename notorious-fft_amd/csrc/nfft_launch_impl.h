@@ -1,0 +1,247 @@
+// nfft_launch_impl.h -- size dispatch from runtime lengths to the compiled
+// kernel instantiations.  Each nfft_k_*.hip translation unit includes this
+// and explicitly instantiates one family for one precision.
+#pragma once
+#include "nfft_internal.h"
+
+namespace nfft {
+
+constexpr size_t kLdsLimit = 160 * 1024;
+
+#define NFFT_POW2_LIST(X)                                                                                     \
+    X(1) X(2) X(4) X(8) X(16) X(32) X(64) X(128) X(256) X(512) X(1024) X(2048) X(4096) X(8192) X(16384)
+
+template <typename T, int L> constexpr bool line_fits()
+{
+    return (size_t)line_lds<T, L, line_lpb<L>()>() * sizeof(Cx<T>) <= kLdsLimit &&
+           Sched<L>::NT * line_lpb<L>() <= 1024;
+}
+
+inline dim3 grid_lines(long long nlines, int lpb)
+{
+    return dim3((unsigned)((nlines + lpb - 1) / lpb));
+}
+
+// transforms split over grid.y (<= 65535) and grid.z
+inline void grid_trans(long long ntrans, unsigned &gy, unsigned &gz)
+{
+    long long y = ntrans < 65535 ? ntrans : 65535;
+    if (y < 1) y = 1;
+    gy = (unsigned)y;
+    gz = (unsigned)((ntrans + y - 1) / y);
+    if (gz < 1) gz = 1;
+}
+
+// ---------------------------------------------------------------------------
+template <typename T> hipError_t line_impl(int L, int dir, const LineArgs &a, hipStream_t s)
+{
+    switch (L) {
+#define NFFT_CASE(LL)                                                                                         \
+    case LL:                                                                                                  \
+        if constexpr (line_fits<T, LL>()) {                                                                   \
+            constexpr int lpb = line_lpb<LL>();                                                               \
+            if (dir < 0) hipLaunchKernelGGL((k_line<T, LL, -1>), grid_lines(a.nlines, lpb),                   \
+                                            dim3(Sched<LL>::NT * lpb), 0, s, a);                              \
+            else hipLaunchKernelGGL((k_line<T, LL, 1>), grid_lines(a.nlines, lpb), dim3(Sched<LL>::NT * lpb), \
+                                    0, s, a);                                                                 \
+            return hipGetLastError();                                                                         \
+        }                                                                                                     \
+        break;
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T> hipError_t blue_line_impl(int M, const BlueArgs &a, hipStream_t s)
+{
+    switch (M) {
+#define NFFT_CASE(LL)                                                                                         \
+    case LL:                                                                                                  \
+        if constexpr (LL >= 2 && line_fits<T, LL>()) {                                                        \
+            constexpr int lpb = line_lpb<LL>();                                                               \
+            hipLaunchKernelGGL((k_blue_line<T, LL>), grid_lines(a.nlines, lpb), dim3(Sched<LL>::NT * lpb), 0, \
+                               s, a);                                                                         \
+            return hipGetLastError();                                                                         \
+        }                                                                                                     \
+        break;
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+template <typename T, int L> constexpr bool col_fits()
+{
+    constexpr int lpb = col_lpb<T, L>();
+    constexpr size_t lds = Sched<L>::S > 1 ? (size_t)L * lpb * sizeof(Cx<T>) : 0;
+    return L >= 2 && L <= 4096 && lds <= kLdsLimit && Sched<L>::NT * lpb <= 1024;
+}
+
+template <typename T> hipError_t col_impl(int L, int dir, int mode, const ColArgs &a, hipStream_t s)
+{
+    switch (L) {
+#define NFFT_CASE(LL)                                                                                         \
+    case LL:                                                                                                  \
+        if constexpr (col_fits<T, LL>()) {                                                                    \
+            constexpr int lpb = col_lpb<T, LL>();                                                             \
+            unsigned gy, gz;                                                                                  \
+            grid_trans(a.ntrans, gy, gz);                                                                     \
+            dim3 g((unsigned)((a.ncols + lpb - 1) / lpb), gy, gz), b(Sched<LL>::NT * lpb);                    \
+            if (mode == COL_PLAIN && dir < 0) hipLaunchKernelGGL((k_col<T, LL, lpb, -1, COL_PLAIN>), g, b, 0, s, a); \
+            else if (mode == COL_PLAIN) hipLaunchKernelGGL((k_col<T, LL, lpb, 1, COL_PLAIN>), g, b, 0, s, a);     \
+            else if (mode == COL_TWIDDLE && dir < 0)                                                          \
+                hipLaunchKernelGGL((k_col<T, LL, lpb, -1, COL_TWIDDLE>), g, b, 0, s, a);                       \
+            else if (mode == COL_TWIDDLE) hipLaunchKernelGGL((k_col<T, LL, lpb, 1, COL_TWIDDLE>), g, b, 0, s, a); \
+            else if (mode == COL_BLUE_IN) hipLaunchKernelGGL((k_col<T, LL, lpb, -1, COL_BLUE_IN>), g, b, 0, s, a); \
+            else if (mode == COL_BLUE_OUT) hipLaunchKernelGGL((k_col<T, LL, lpb, 1, COL_BLUE_OUT>), g, b, 0, s, a); \
+            else return hipErrorInvalidValue;                                                                 \
+            return hipGetLastError();                                                                         \
+        }                                                                                                     \
+        break;
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T, int L> constexpr bool rowT_fits()
+{
+    constexpr int lpb = rowT_lpb<T, L>();
+    return L >= 2 && L <= (sizeof(T) == 4 ? 2048 : 1024) &&
+           (size_t)rowT_lds<T, L, lpb>() * sizeof(Cx<T>) <= kLdsLimit && Sched<L>::NT * lpb <= 1024;
+}
+
+template <typename T> hipError_t rowT_impl(int L, int dir, const RowTArgs &a, hipStream_t s)
+{
+    switch (L) {
+#define NFFT_CASE(LL)                                                                                         \
+    case LL:                                                                                                  \
+        if constexpr (rowT_fits<T, LL>()) {                                                                   \
+            constexpr int lpb = rowT_lpb<T, LL>();                                                            \
+            unsigned gy, gz;                                                                                  \
+            grid_trans(a.ntrans, gy, gz);                                                                     \
+            dim3 g((unsigned)((a.nrows + lpb - 1) / lpb), gy, gz), b(Sched<LL>::NT * lpb);                    \
+            if (dir < 0) hipLaunchKernelGGL((k_rowT<T, LL, lpb, -1>), g, b, 0, s, a);                         \
+            else hipLaunchKernelGGL((k_rowT<T, LL, lpb, 1>), g, b, 0, s, a);                                  \
+            return hipGetLastError();                                                                         \
+        }                                                                                                     \
+        break;
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T, int L> constexpr bool bluerow_fits()
+{
+    constexpr int lpb = bluerow_lpb<T, L>();
+    return L >= 2 && L <= (sizeof(T) == 4 ? 4096 : 2048) &&
+           (size_t)line_lds<T, L, lpb>() * sizeof(Cx<T>) <= kLdsLimit && Sched<L>::NT * lpb <= 1024;
+}
+
+template <typename T> hipError_t blue_row_impl(int L, const BlueRowArgs &a, hipStream_t s)
+{
+    switch (L) {
+#define NFFT_CASE(LL)                                                                                         \
+    case LL:                                                                                                  \
+        if constexpr (bluerow_fits<T, LL>()) {                                                                \
+            constexpr int lpb = bluerow_lpb<T, LL>();                                                         \
+            unsigned gy, gz;                                                                                  \
+            grid_trans(a.ntrans, gy, gz);                                                                     \
+            dim3 g((unsigned)((a.nrows + lpb - 1) / lpb), gy, gz), b(Sched<LL>::NT * lpb);                    \
+            hipLaunchKernelGGL((k_blue_row<T, LL, lpb>), g, b, 0, s, a);                                      \
+            return hipGetLastError();                                                                         \
+        }                                                                                                     \
+        break;
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+template <typename T> hipError_t r2c_impl(int L, bool inverse, const RtArgs &a, hipStream_t s)
+{
+    switch (L) {
+#define NFFT_CASE(LL)                                                                                         \
+    case LL:                                                                                                  \
+        if constexpr (line_fits<T, LL>()) {                                                                   \
+            constexpr int lpb = line_lpb<LL>();                                                               \
+            if (!inverse) hipLaunchKernelGGL((k_r2c_line<T, LL>), grid_lines(a.nlines, lpb),                  \
+                                             dim3(Sched<LL>::NT * lpb), 0, s, a);                             \
+            else hipLaunchKernelGGL((k_c2r_line<T, LL>), grid_lines(a.nlines, lpb),                           \
+                                    dim3(Sched<LL>::NT * lpb), 0, s, a);                                      \
+            return hipGetLastError();                                                                         \
+        }                                                                                                     \
+        break;
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T> hipError_t t23_impl(int L, int op, const RtArgs &a, hipStream_t s)
+{
+    switch (L) {
+#define NFFT_CASE(LL)                                                                                         \
+    case LL:                                                                                                  \
+        if constexpr (line_fits<T, LL>()) {                                                                   \
+            constexpr int lpb = line_lpb<LL>();                                                               \
+            dim3 g = grid_lines(a.nlines, lpb), b(Sched<LL>::NT * lpb);                                       \
+            switch (op) {                                                                                     \
+            case T_DCT2: hipLaunchKernelGGL((k_t23_line<T, LL, T_DCT2>), g, b, 0, s, a); break;             \
+            case T_DST2: hipLaunchKernelGGL((k_t23_line<T, LL, T_DST2>), g, b, 0, s, a); break;             \
+            case T_DCT3: hipLaunchKernelGGL((k_t23_line<T, LL, T_DCT3>), g, b, 0, s, a); break;             \
+            case T_DST3: hipLaunchKernelGGL((k_t23_line<T, LL, T_DST3>), g, b, 0, s, a); break;             \
+            default: return hipErrorInvalidValue;                                                             \
+            }                                                                                                 \
+            return hipGetLastError();                                                                         \
+        }                                                                                                     \
+        break;
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+template <typename T> hipError_t t4_impl(int L, int op, const RtArgs &a, hipStream_t s)
+{
+    switch (L) {
+#define NFFT_CASE(LL)                                                                                         \
+    case LL:                                                                                                  \
+        if constexpr (line_fits<T, LL>()) {                                                                   \
+            constexpr int lpb = line_lpb<LL>();                                                               \
+            dim3 g = grid_lines(a.nlines, lpb), b(Sched<LL>::NT * lpb);                                       \
+            if (op == T_DCT4) hipLaunchKernelGGL((k_t4_line<T, LL, T_DCT4>), g, b, 0, s, a);                 \
+            else if (op == T_DST4) hipLaunchKernelGGL((k_t4_line<T, LL, T_DST4>), g, b, 0, s, a);            \
+            else return hipErrorInvalidValue;                                                                 \
+            return hipGetLastError();                                                                         \
+        }                                                                                                     \
+        break;
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
+// explicit instantiations live in the nfft_k_*.hip files
+extern template hipError_t line_impl<float>(int, int, const LineArgs &, hipStream_t);
+extern template hipError_t line_impl<double>(int, int, const LineArgs &, hipStream_t);
+extern template hipError_t blue_line_impl<float>(int, const BlueArgs &, hipStream_t);
+extern template hipError_t blue_line_impl<double>(int, const BlueArgs &, hipStream_t);
+extern template hipError_t col_impl<float>(int, int, int, const ColArgs &, hipStream_t);
+extern template hipError_t col_impl<double>(int, int, int, const ColArgs &, hipStream_t);
+extern template hipError_t rowT_impl<float>(int, int, const RowTArgs &, hipStream_t);
+extern template hipError_t rowT_impl<double>(int, int, const RowTArgs &, hipStream_t);
+extern template hipError_t blue_row_impl<float>(int, const BlueRowArgs &, hipStream_t);
+extern template hipError_t blue_row_impl<double>(int, const BlueRowArgs &, hipStream_t);
+extern template hipError_t r2c_impl<float>(int, bool, const RtArgs &, hipStream_t);
+extern template hipError_t r2c_impl<double>(int, bool, const RtArgs &, hipStream_t);
+extern template hipError_t t23_impl<float>(int, int, const RtArgs &, hipStream_t);
+extern template hipError_t t23_impl<double>(int, int, const RtArgs &, hipStream_t);
+extern template hipError_t t4_impl<float>(int, int, const RtArgs &, hipStream_t);
+extern template hipError_t t4_impl<double>(int, int, const RtArgs &, hipStream_t);
+
+}  // namespace nfft

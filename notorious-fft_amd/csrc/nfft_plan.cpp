@@ -1,0 +1,1012 @@
+// nfft_plan.cpp -- planner and executor behind the C ABI.
+//
+// Replaces the reference's planner (src/06_plan.h:28-246: power-of-two vs
+// Bluestein, one slab of tables per plan) and its aux tree
+// (src/07_legacy.h:1389-1700) with device-resident plans: tables are
+// generated in FP64 on the GPU and cached per device, scratch lives in one
+// HBM allocation per plan, and every transform is a short sequence of kernel
+// launches on the caller's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/notorious_fft_hip.h"
+#include "nfft_internal.h"
+#include "nfft_plan.h"
+
+namespace nfft {
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string t_err;
+void set_error(const std::string &msg) { t_err = msg; }
+void clear_error() { t_err.clear(); }
+const char *last_error() { return t_err.c_str(); }
+
+static bool hip_ok(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return true;
+    set_error(std::string(what) + ": " + hipGetErrorString(e));
+    return false;
+}
+
+static bool is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
+static int ilog2ll(long long n)
+{
+    int l = 0;
+    while ((1LL << l) < n) l++;
+    return l;
+}
+
+// ---------------------------------------------------------------------------
+// per-device table cache
+// ---------------------------------------------------------------------------
+namespace {
+struct Cache {
+    std::mutex mu;
+    std::map<std::string, void *> tabs;
+    hipStream_t st = nullptr;
+};
+Cache g_cache[64];
+
+Cache &cur_cache()
+{
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    return g_cache[dev & 63];
+}
+}  // namespace
+
+hipStream_t setup_stream()
+{
+    Cache &c = cur_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    if (!c.st) {
+        if (hipStreamCreateWithFlags(&c.st, hipStreamNonBlocking) != hipSuccess) c.st = nullptr;
+    }
+    return c.st;
+}
+
+template <typename Gen> static void *cache_get(const std::string &key, size_t bytes, Gen gen)
+{
+    hipStream_t st = setup_stream();
+    if (!st) {
+        set_error("cannot create the setup stream");
+        return nullptr;
+    }
+    Cache &c = cur_cache();
+    std::lock_guard<std::mutex> lk(c.mu);
+    auto it = c.tabs.find(key);
+    if (it != c.tabs.end()) return it->second;
+    void *p = nullptr;
+    if (!hip_ok(hipMalloc(&p, bytes < 16 ? 16 : bytes), "hipMalloc(table)")) return nullptr;
+    if (!hip_ok(gen(p, st), "table generation") || !hip_ok(hipStreamSynchronize(st), "table generation")) {
+        (void)hipFree(p);
+        return nullptr;
+    }
+    c.tabs[key] = p;
+    return p;
+}
+
+struct StageInfo {
+    int S, R0, twn;
+    int ns(int s) const { return s == 0 ? 1 : R0 << (4 * (s - 1)); }
+    int twoff(int s) const
+    {
+        int o = 0;
+        for (int t = 1; t < s; t++) o += ns(t) * 16;
+        return o;
+    }
+};
+static StageInfo stage_info(int L)
+{
+    StageInfo si;
+    const int lg = ilog2ll(L);
+    si.S = lg <= 4 ? 1 : (lg + 3) / 4;
+    si.R0 = L >> (4 * (si.S - 1));
+    si.twn = si.S <= 1 ? 0 : si.twoff(si.S);
+    return si;
+}
+
+const void *stage_table(int prec, int L)
+{
+    const StageInfo si = stage_info(L);
+    const std::string key = "stage/" + std::to_string(prec) + "/" + std::to_string(L);
+    return cache_get(key, (size_t)std::max(1, si.twn) * cx_bytes(prec), [&](void *p, hipStream_t st) {
+        for (int s = 1; s < si.S; s++) {
+            const long long cnt = (long long)si.ns(s) * 16;
+            const int off = si.twoff(s);
+            hipError_t e = launch_gen_phase(prec == F64 ? (double2 *)p + off : nullptr,
+                                            prec == F32 ? (float2 *)p + off : nullptr, cnt, 1, 0, 0, cnt, st);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    });
+}
+
+const void *phase_table(int prec, long long count, long long a, long long c, long long den)
+{
+    const std::string key = "ph/" + std::to_string(prec) + "/" + std::to_string(count) + "/" + std::to_string(a) +
+                            "/" + std::to_string(c) + "/" + std::to_string(den);
+    return cache_get(key, (size_t)std::max(1LL, count) * cx_bytes(prec), [&](void *p, hipStream_t st) {
+        return launch_gen_phase(prec == F64 ? (double2 *)p : nullptr, prec == F32 ? (float2 *)p : nullptr, count,
+                                0, a, c, den, st);
+    });
+}
+
+bool tw2_table(long long n, Tw2 *out)
+{
+    const int lg = ilog2ll(n);
+    const int sh = (lg + 1) / 2;
+    const long long lo_n = 1LL << sh, hi_n = (n + lo_n - 1) >> sh;
+    const void *hi = phase_table(F64, hi_n, lo_n, 0, n);
+    const void *lo = phase_table(F64, lo_n, 1, 0, n);
+    if (!hi || !lo) return false;
+    out->hi = (const double2 *)hi;
+    out->lo = (const double2 *)lo;
+    out->sh = sh;
+    out->mask = (int)(lo_n - 1);
+    return true;
+}
+
+const void *chirp_table(int prec, long long n, const double2 **chirp64)
+{
+    auto gen = [&](int pr) {
+        const std::string key = "chirp/" + std::to_string(pr) + "/" + std::to_string(n);
+        return cache_get(key, (size_t)n * cx_bytes(pr), [&](void *p, hipStream_t st) {
+            return launch_gen_phase(pr == F64 ? (double2 *)p : nullptr, pr == F32 ? (float2 *)p : nullptr, n, 2, 0,
+                                    0, 2 * n, st);
+        });
+    };
+    if (chirp64) *chirp64 = (const double2 *)gen(F64);
+    return gen(prec);
+}
+
+// ---------------------------------------------------------------------------
+// 1D complex engine
+// ---------------------------------------------------------------------------
+static int env_int(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return v && *v ? atoi(v) : dflt;
+}
+
+bool C2C::init(int n_, int prec_)
+{
+    n = n_;
+    prec = prec_;
+    if (n < 1) {
+        set_error("transform length must be >= 1");
+        return false;
+    }
+    if (is_pow2(n) && n <= line_max(prec)) {
+        algo = ALG_LINE;
+        L = n;
+        tw_a = stage_table(prec, L);
+        return tw_a != nullptr;
+    }
+    if (is_pow2(n)) {
+        const int lg = ilog2ll(n);
+        int lg2 = std::min(lg / 2, ilog2ll(rowT_max(prec)));
+        const int forced = env_int("NFFT_FOUR_N2", 0);
+        if (forced > 0 && is_pow2(forced) && forced <= rowT_max(prec) && n / forced <= col_max(prec) &&
+            n / forced >= 2)
+            lg2 = ilog2ll(forced);
+        n2 = 1 << lg2;
+        n1 = n / n2;
+        if (n1 > col_max(prec)) {
+            set_error("power-of-two length " + std::to_string(n) + " exceeds this build's limit");
+            return false;
+        }
+        algo = ALG_FOUR;
+        tw_a = stage_table(prec, n1);
+        tw_b = stage_table(prec, n2);
+        if (!tw_a || !tw_b || !tw2_table(n, &tw2)) return false;
+        scratch_per_line = n;
+        return true;
+    }
+    // Bluestein
+    m = 1;
+    while (m < 2LL * n - 1) m <<= 1;
+    const double2 *chirp64 = nullptr;
+    chirp = chirp_table(prec, n, &chirp64);
+    if (!chirp || !chirp64) return false;
+    if (m <= line_max(prec)) {
+        algo = ALG_BLUE1;
+        L = m;
+        tw_a = stage_table(prec, m);
+        if (!tw_a) return false;
+    } else {
+        const int lg = ilog2ll(m);
+        const int lg2 = std::min(lg / 2, ilog2ll(bluerow_max(prec)));
+        m2 = 1 << lg2;
+        m1 = m / m2;
+        if (m1 > col_max(prec)) {
+            set_error("Bluestein length " + std::to_string(n) + " exceeds this build's limit");
+            return false;
+        }
+        algo = ALG_BLUE3;
+        tw_a = stage_table(prec, m1);
+        tw_b = stage_table(prec, m2);
+        if (!tw_a || !tw_b || !tw2_table(m, &tw2)) return false;
+        scratch_per_line = m;
+    }
+    // filter spectrum H = FFT_m(h) / m, computed in FP64 by this engine
+    C2C h64;
+    if (!h64.init(m, F64)) return false;
+    hipStream_t st = setup_stream();
+    void *hbuf = nullptr, *Hbuf = nullptr, *sbuf = nullptr;
+    bool ok = hip_ok(hipMalloc(&hbuf, (size_t)m * 16), "hipMalloc") &&
+              hip_ok(hipMalloc(&Hbuf, (size_t)m * 16), "hipMalloc") &&
+              hip_ok(hipMalloc(&H, (size_t)m * cx_bytes(prec)), "hipMalloc");
+    if (ok && h64.scratch_per_line) ok = hip_ok(hipMalloc(&sbuf, (size_t)h64.scratch_per_line * 16), "hipMalloc");
+    ok = ok && hip_ok(launch_blue_filter((double2 *)hbuf, chirp64, n, m, st), "filter") &&
+         hip_ok(h64.run(hbuf, Hbuf, 1, m, m, -1, 1.0, sbuf, 1, st), "filter FFT") &&
+         hip_ok(launch_blue_hprep((const double2 *)Hbuf, H, prec, m, algo == ALG_BLUE3 ? m1 : 0, m2, 1.0 / m, st),
+                "filter prep") &&
+         hip_ok(hipStreamSynchronize(st), "filter sync");
+    (void)hipFree(hbuf);
+    (void)hipFree(Hbuf);
+    if (sbuf) (void)hipFree(sbuf);
+    if (!ok) {
+        if (H) (void)hipFree(H);
+        H = nullptr;
+    }
+    return ok;
+}
+
+void C2C::release()
+{
+    if (H) (void)hipFree(H);
+    H = nullptr;
+}
+
+std::string C2C::describe() const
+{
+    char b[128];
+    switch (algo) {
+    case ALG_LINE: snprintf(b, sizeof b, "line %d", n); break;
+    case ALG_FOUR: snprintf(b, sizeof b, "four-step %dx%d", n1, n2); break;
+    case ALG_BLUE1: snprintf(b, sizeof b, "bluestein N=%d M=%d single-pass", n, m); break;
+    case ALG_BLUE3: snprintf(b, sizeof b, "bluestein N=%d M=%d (%dx%d)", n, m, m1, m2); break;
+    default: snprintf(b, sizeof b, "n=%d", n);
+    }
+    return b;
+}
+
+hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_dist, long long out_dist, int dir,
+                    double scale, void *scratch, long long scratch_lines, hipStream_t s) const
+{
+    if (nlines <= 0) return hipSuccess;
+    const size_t cb = cx_bytes(prec);
+    switch (algo) {
+    case ALG_LINE: {
+        LineArgs a{in, out, nlines, in_dist, out_dist, tw_a, scale};
+        return launch_line(prec, L, dir, a, s);
+    }
+    case ALG_BLUE1: {
+        BlueArgs a{in, out, nlines, in_dist, out_dist, n, tw_a, chirp, H, dir > 0 ? 1 : 0, scale};
+        return launch_blue_line(prec, m, a, s);
+    }
+    case ALG_FOUR: {
+        if (scratch_lines < 1 || !scratch) return hipErrorInvalidValue;
+        for (long long l0 = 0; l0 < nlines; l0 += scratch_lines) {
+            const long long cnt = std::min(scratch_lines, nlines - l0);
+            ColArgs c{};
+            c.in = (const char *)in + (size_t)l0 * in_dist * cb;
+            c.out = scratch;
+            c.ncols = n2;
+            c.S = n2;
+            c.in_dist = in_dist;
+            c.out_dist = n;
+            c.ntrans = cnt;
+            c.tw = tw_a;
+            c.tw2 = tw2;
+            c.scale = 1.0;
+            hipError_t e = launch_col(prec, n1, dir, COL_TWIDDLE, c, s);
+            if (e != hipSuccess) return e;
+            RowTArgs r{scratch, (char *)out + (size_t)l0 * out_dist * cb, n1, n, out_dist, cnt, tw_b, scale};
+            e = launch_rowT(prec, n2, dir, r, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    case ALG_BLUE3: {
+        if (scratch_lines < 1 || !scratch) return hipErrorInvalidValue;
+        for (long long l0 = 0; l0 < nlines; l0 += scratch_lines) {
+            const long long cnt = std::min(scratch_lines, nlines - l0);
+            ColArgs c{};
+            c.in = (const char *)in + (size_t)l0 * in_dist * cb;
+            c.out = scratch;
+            c.ncols = m2;
+            c.S = m2;
+            c.in_dist = in_dist;
+            c.out_dist = m;
+            c.ntrans = cnt;
+            c.tw = tw_a;
+            c.tw2 = tw2;
+            c.chirp = chirp;
+            c.nv = n;
+            c.conj = dir > 0 ? 1 : 0;
+            c.scale = 1.0;
+            hipError_t e = launch_col(prec, m1, -1, COL_BLUE_IN, c, s);
+            if (e != hipSuccess) return e;
+            BlueRowArgs r{scratch, m1, m, cnt, tw_b, H, tw2};
+            e = launch_blue_row(prec, m2, r, s);
+            if (e != hipSuccess) return e;
+            ColArgs o{};
+            o.in = scratch;
+            o.out = (char *)out + (size_t)l0 * out_dist * cb;
+            o.ncols = m2;
+            o.S = m2;
+            o.in_dist = m;
+            o.out_dist = out_dist;
+            o.ntrans = cnt;
+            o.tw = tw_a;
+            o.tw2 = tw2;
+            o.chirp = chirp;
+            o.nv = n;
+            o.conj = dir > 0 ? 1 : 0;
+            o.scale = scale;
+            e = launch_col(prec, m1, 1, COL_BLUE_OUT, o, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t c2c_axis(const C2C &e, void *buf, long long A, long long B, int dir, double scale, void *scratch,
+                    long long scratch_elems, hipStream_t s)
+{
+    const int n = e.n;
+    if (n == 1) {
+        if (scale == 1.0) return hipSuccess;
+        return launch_scale_real(buf, 2 * A * B, e.prec, scale, s);
+    }
+    if (B == 1) {
+        const long long fit = e.scratch_per_line ? scratch_elems / e.scratch_per_line : 0;
+        return e.run(buf, buf, A, n, n, dir, scale, scratch, fit, s);
+    }
+    if (e.algo == ALG_LINE && n <= col_max(e.prec) && B <= 0x7fffffffLL) {
+        ColArgs c{};
+        c.in = buf;
+        c.out = buf;
+        c.ncols = (int)B;
+        c.S = B;
+        c.in_dist = (long long)n * B;
+        c.out_dist = (long long)n * B;
+        c.ntrans = A;
+        c.tw = e.tw_a;
+        c.scale = scale;
+        return launch_col(e.prec, n, dir, COL_PLAIN, c, s);
+    }
+    // general axis: transpose to make it contiguous, transform, transpose back
+    const long long tot = A * n * B;
+    if (scratch_elems < tot) return hipErrorInvalidValue;
+    const size_t cb = cx_bytes(e.prec);
+    hipError_t err = launch_transpose(buf, scratch, A, n, B, (int)cb, s);
+    if (err != hipSuccess) return err;
+    const long long rest = scratch_elems - tot;
+    const long long fit = e.scratch_per_line ? rest / e.scratch_per_line : 0;
+    err = e.run(scratch, scratch, A * B, n, n, dir, scale, (char *)scratch + (size_t)tot * cb, fit, s);
+    if (err != hipSuccess) return err;
+    return launch_transpose(scratch, buf, A, B, n, (int)cb, s);
+}
+
+// ---------------------------------------------------------------------------
+// real (realdft) engine along contiguous lines of N reals
+// ---------------------------------------------------------------------------
+bool R2C::init(int N_, int prec_)
+{
+    N = N_;
+    prec = prec_;
+    if (!is_pow2(N)) {
+        set_error("realdft needs a power-of-two length");
+        return false;
+    }
+    if (N == 1) return true;
+    L = N / 2;
+    rtw = phase_table(prec, L, 1, 0, N);
+    if (!rtw) return false;
+    if (L <= line_max(prec)) {
+        single = true;
+        tw = stage_table(prec, L);
+        return tw != nullptr;
+    }
+    single = false;
+    if (!inner.init(L, prec)) return false;
+    scratch_per_line = L + inner.scratch_per_line;
+    return true;
+}
+
+void R2C::release() { inner.release(); }
+
+hipError_t R2C::fwd(const void *in, void *out, long long nlines, long long in_dist, long long out_dist, double scale,
+                    int dbl, void *scratch, long long scratch_elems, hipStream_t s) const
+{
+    if (nlines <= 0) return hipSuccess;
+    if (N == 1) {
+        EwArgs a{in, out, nlines, in_dist, out_dist, 0, nullptr, nullptr, scale, 1.0, 0, 1};
+        return launch_ew(prec, EW_N1, a, 1, s);
+    }
+    if (single) {
+        RtArgs a{in, out, nlines, in_dist, out_dist, tw, rtw, nullptr, scale, 1.0, dbl};
+        return launch_r2c(prec, L, a, s);
+    }
+    const long long chunk = scratch_elems / scratch_per_line;
+    if (chunk < 1) return hipErrorInvalidValue;
+    const size_t cb = cx_bytes(prec), rb = re_bytes(prec);
+    void *Z = scratch;
+    void *isc = (char *)scratch + (size_t)chunk * L * cb;
+    for (long long l0 = 0; l0 < nlines; l0 += chunk) {
+        const long long cnt = std::min(chunk, nlines - l0);
+        hipError_t e = inner.run((const char *)in + (size_t)l0 * in_dist * rb, Z, cnt, in_dist / 2, L, -1, 1.0, isc,
+                                 chunk, s);
+        if (e != hipSuccess) return e;
+        EwArgs a{Z, (char *)out + (size_t)l0 * out_dist * cb, cnt, L, out_dist, L, rtw, nullptr, scale, 1.0, dbl, 0};
+        e = launch_ew(prec, EW_UNPACK, a, L, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t R2C::inv(const void *in, void *out, long long nlines, long long in_dist, long long out_dist, double scale,
+                    int half, void *scratch, long long scratch_elems, hipStream_t s) const
+{
+    if (nlines <= 0) return hipSuccess;
+    if (N == 1) {
+        EwArgs a{in, out, nlines, in_dist, out_dist, 0, nullptr, nullptr, scale, 1.0, 0, 2};
+        return launch_ew(prec, EW_N1, a, 1, s);
+    }
+    if (single) {
+        RtArgs a{in, out, nlines, in_dist, out_dist, tw, rtw, nullptr, scale, 1.0, half};
+        return launch_c2r(prec, L, a, s);
+    }
+    const long long chunk = scratch_elems / scratch_per_line;
+    if (chunk < 1) return hipErrorInvalidValue;
+    const size_t cb = cx_bytes(prec), rb = re_bytes(prec);
+    void *Z = scratch;
+    void *isc = (char *)scratch + (size_t)chunk * L * cb;
+    for (long long l0 = 0; l0 < nlines; l0 += chunk) {
+        const long long cnt = std::min(chunk, nlines - l0);
+        EwArgs a{(const char *)in + (size_t)l0 * in_dist * cb, Z, cnt, in_dist, L, L, rtw, nullptr, 1.0, 1.0, half, 0};
+        hipError_t e = launch_ew(prec, EW_PACK, a, L, s);
+        if (e != hipSuccess) return e;
+        e = inner.run(Z, (char *)out + (size_t)l0 * out_dist * rb, cnt, L, out_dist / 2, 1, scale, isc, chunk, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ---------------------------------------------------------------------------
+// trig engine (DCT/DST 2/3 or 4) along contiguous lines of N reals
+// ---------------------------------------------------------------------------
+bool Trig::init(int N_, int prec_, int t4_)
+{
+    N = N_;
+    prec = prec_;
+    t4 = t4_;
+    if (!is_pow2(N)) {
+        set_error("trigonometric transforms need a power-of-two length");
+        return false;
+    }
+    if (N == 1) return true;
+    L = N / 2;
+    if (!t4) {
+        dw = phase_table(prec, L, 1, 0, 4LL * N);   // e^{-i pi k/(2N)}
+        rw = phase_table(prec, L, 1, 0, N);         // e^{-2 pi i k/N}
+        if (!dw || !rw) return false;
+    } else {
+        pre = phase_table(prec, L, 1, 0, 2LL * N);      // e^{-i pi n/N}
+        post = phase_table(prec, N, 2, 1, 8LL * N);     // e^{-i pi (2n+1)/(4N)}
+        if (!pre || !post) return false;
+    }
+    if (L <= line_max(prec)) {
+        single = true;
+        tw = stage_table(prec, L);
+        return tw != nullptr;
+    }
+    single = false;
+    if (!t4) {
+        if (!real.init(N, prec)) return false;
+        // pre buffer (L+1 complex) + post buffer (L+1 complex) + realdft scratch
+        scratch_per_line = 2 * (L + 1) + real.scratch_per_line;
+    } else {
+        if (!cplx.init(L, prec)) return false;
+        scratch_per_line = 2 * L + cplx.scratch_per_line;
+    }
+    return true;
+}
+
+void Trig::release()
+{
+    real.release();
+    cplx.release();
+}
+
+hipError_t Trig::run(int op, const void *in, void *out, long long nlines, long long in_dist, long long out_dist,
+                     double scale, double w0, void *scratch, long long scratch_elems, hipStream_t s) const
+{
+    if (nlines <= 0) return hipSuccess;
+    const int top = op - NFFT_OP_DCT2;  // T_DCT2..T_DST4
+    if (N == 1) {
+        double f = 1.0;
+        if (top == T_DCT2 || top == T_DST2) f = 2.0 * w0;
+        else if (top == T_DCT3 || top == T_DST3) f = w0;
+        else f = 1.41421356237309504880;
+        EwArgs a{in, out, nlines, in_dist, out_dist, 0, nullptr, nullptr, f * scale, 1.0, 0, 3};
+        return launch_ew(prec, EW_N1, a, 1, s);
+    }
+    if (single) {
+        if (!t4) {
+            RtArgs a{in, out, nlines, in_dist, out_dist, tw, dw, rw, scale, w0, 0};
+            return launch_t23(prec, L, top, a, s);
+        }
+        RtArgs a{in, out, nlines, in_dist, out_dist, tw, pre, post, scale, 1.0, 0};
+        return launch_t4(prec, L, top, a, s);
+    }
+    const long long chunk = scratch_elems / scratch_per_line;
+    if (chunk < 1) return hipErrorInvalidValue;
+    const size_t cb = cx_bytes(prec), rb = re_bytes(prec);
+    for (long long l0 = 0; l0 < nlines; l0 += chunk) {
+        const long long cnt = std::min(chunk, nlines - l0);
+        const void *src = (const char *)in + (size_t)l0 * in_dist * rb;
+        void *dst = (char *)out + (size_t)l0 * out_dist * rb;
+        hipError_t e;
+        if (!t4) {
+            void *A = scratch;
+            void *B = (char *)scratch + (size_t)chunk * (L + 1) * cb;
+            void *rs = (char *)scratch + (size_t)chunk * 2 * (L + 1) * cb;
+            const long long rsz = scratch_elems - chunk * 2 * (L + 1);
+            EwArgs pa{src, A, cnt, in_dist, L + 1, L, dw, rw, 1.0, w0, 0, top};
+            e = launch_ew(prec, EW_TRIG_PRE, pa, L + 1, s);
+            if (e != hipSuccess) return e;
+            if (top == T_DCT2 || top == T_DST2) {
+                // A holds the packed reorder (N reals per line at stride 2(L+1))
+                e = real.fwd(A, B, cnt, 2 * (L + 1), L + 1, 1.0, 0, rs, rsz, s);
+                if (e != hipSuccess) return e;
+                EwArgs qa{B, dst, cnt, L + 1, out_dist, L, dw, rw, scale, w0, 0, top};
+                e = launch_ew(prec, EW_TRIG_POST, qa, L, s);
+            } else {
+                e = real.inv(A, B, cnt, L + 1, 2 * (L + 1), 1.0, 0, rs, rsz, s);
+                if (e != hipSuccess) return e;
+                EwArgs qa{B, dst, cnt, 2 * (L + 1), out_dist, L, dw, rw, scale, w0, 0, top};
+                e = launch_ew(prec, EW_TRIG_POST, qa, L, s);
+            }
+        } else {
+            void *A = scratch;
+            void *B = (char *)scratch + (size_t)chunk * L * cb;
+            void *cs = (char *)scratch + (size_t)chunk * 2 * L * cb;
+            const long long csz = scratch_elems - chunk * 2 * L;
+            const long long cfit = cplx.scratch_per_line ? csz / cplx.scratch_per_line : 0;
+            EwArgs pa{src, A, cnt, in_dist, L, L, pre, post, 1.0, 1.0, 0, top};
+            e = launch_ew(prec, EW_TRIG_PRE, pa, L, s);
+            if (e != hipSuccess) return e;
+            e = cplx.run(A, B, cnt, L, L, -1, 1.0, cs, cfit, s);
+            if (e != hipSuccess) return e;
+            EwArgs qa{B, dst, cnt, L, out_dist, L, pre, post, scale, 1.0, 0, top};
+            e = launch_ew(prec, EW_TRIG_POST, qa, L, s);
+        }
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace nfft
+
+// ===========================================================================
+// plans
+// ===========================================================================
+using namespace nfft;
+
+static long long prod(const std::vector<int> &v, size_t b, size_t e)
+{
+    long long p = 1;
+    for (size_t i = b; i < e; i++) p *= v[i];
+    return p;
+}
+
+static const long long kChunkBytesDefault = 64LL << 20;
+
+static long long chunk_lines(long long nlines, long long per_line_elems, size_t eb)
+{
+    if (per_line_elems <= 0) return 0;
+    long long target = (long long)env_int("NFFT_CHUNK_MB", 0) << 20;
+    if (target <= 0) target = kChunkBytesDefault;
+    long long c = target / (per_line_elems * (long long)eb);
+    c = std::max(1LL, std::min(c, nlines));
+    return c;
+}
+
+static double axis_inv_scale(int norm, int n)
+{
+    if (norm == NFFT_NORM_NONE) return 1.0;
+    if (norm == NFFT_NORM_INVERSE) return 1.0 / n;
+    return is_pow2(n) ? 1.0 : 1.0 / n;  // reference behaviour
+}
+
+bool nfft_plan::init(int kind_, int prec_, int d_, const int *dims_, long long batch_, int norm_)
+{
+    kind = kind_;
+    prec = prec_;
+    d = d_;
+    norm = norm_;
+    batch = batch_;
+    dims.assign(dims_, dims_ + d);
+    P = prod(dims, 0, d);
+    const size_t cb = cx_bytes(prec), rb = re_bytes(prec);
+    long long full = 0;   // elements for a full-array copy (transposes / inverse copies)
+    long long eng = 0;    // engine scratch elements
+    const int last = dims[d - 1];
+    std::string dsc;
+    if (kind == NFFT_KIND_DFT) {
+        ax.resize(d);
+        for (int i = 0; i < d; i++) {
+            if (!ax[i].init(dims[i], prec)) return false;
+            dsc += (i ? " | " : "") + ax[i].describe();
+        }
+        const long long lines = batch * P / last;
+        eng = chunk_lines(lines, ax[d - 1].scratch_per_line, cb) * ax[d - 1].scratch_per_line;
+        for (int i = 0; i + 1 < d; i++) {
+            const C2C &e = ax[i];
+            const bool direct = dims[i] == 1 || (e.algo == ALG_LINE && dims[i] <= col_max(prec));
+            if (!direct) {
+                full = std::max(full, batch * P);
+                const long long lines_i = batch * P / dims[i];
+                eng = std::max(eng, chunk_lines(lines_i, e.scratch_per_line, cb) * e.scratch_per_line);
+            }
+        }
+        work_bytes = (size_t)(full + eng) * cb;
+    } else if (kind == NFFT_KIND_REALDFT) {
+        if (!real.init(last, prec)) return false;
+        dsc = "realdft " + std::to_string(last) + (real.single ? " single-pass" : " via " + real.inner.describe());
+        const long long H = last / 2 + 1;
+        const long long Pl = P / last;  // leading product
+        ax.resize(d - 1);
+        for (int i = 0; i + 1 < d; i++) {
+            if (!ax[i].init(dims[i], prec)) return false;
+            dsc += " | " + ax[i].describe();
+        }
+        const long long lines = batch * Pl;
+        long long e_real = chunk_lines(lines, real.scratch_per_line, cb) * real.scratch_per_line;
+        eng = e_real;
+        long long cfull = 0;
+        if (d > 1) cfull = batch * Pl * H;  // copy of the spectrum for the inverse
+        for (int i = 0; i + 1 < d; i++) {
+            const C2C &e = ax[i];
+            const bool direct = dims[i] == 1 || (e.algo == ALG_LINE && dims[i] <= col_max(prec));
+            if (!direct) {
+                const long long lines_i = batch * Pl * H / dims[i];
+                eng = std::max(eng, batch * Pl * H + chunk_lines(lines_i, e.scratch_per_line, cb) * e.scratch_per_line);
+            }
+        }
+        full = cfull;
+        work_bytes = (size_t)(full + eng) * cb;
+    } else {
+        const int t4 = kind == NFFT_KIND_T4;
+        tax.resize(d);
+        for (int i = 0; i < d; i++) {
+            if (!tax[i].init(dims[i], prec, t4)) return false;
+        }
+        // the reference transforms every axis except a trailing run of unit
+        // axes (src/07_legacy.h:822-861: N1 == 1 short-cut at each level)
+        n_active = d;
+        while (n_active > 1 && dims[n_active - 1] == 1) n_active--;
+        dsc = std::string(t4 ? "dct4/dst4 " : "dct2/3 dst2/3 ") + std::to_string(P) +
+              (d > 1 ? " separable" : (tax[0].single ? " single-pass" : " multi-pass"));
+        long long e_max = 0;
+        for (int i = 0; i < n_active; i++) {
+            const long long lines_i = batch * P / dims[i];
+            e_max = std::max(e_max, chunk_lines(lines_i, tax[i].scratch_per_line, rb) * tax[i].scratch_per_line);
+        }
+        // in reals: full-array transposes (2 copies) + the engine scratch (complex)
+        full = n_active > 1 || d > 1 ? 2 * batch * P : 0;
+        work_bytes = (size_t)full * rb + (size_t)e_max * cb;
+    }
+    desc = dsc;
+    if (work_bytes) {
+        if (hipMalloc(&work, work_bytes) != hipSuccess) {
+            set_error("hipMalloc of " + std::to_string(work_bytes) + " workspace bytes failed");
+            work = nullptr;
+            return false;
+        }
+    }
+    return true;
+}
+
+void nfft_plan::release()
+{
+    for (auto &e : ax) e.release();
+    real.release();
+    for (auto &t : tax) t.release();
+    if (work) (void)hipFree(work);
+    work = nullptr;
+}
+
+// complex N-D over dims[0..nd-1] of `lines` stacked arrays, in place on buf
+// except for the first (last-axis) pass which reads `in`.
+static hipError_t dft_nd(const nfft_plan &p, const std::vector<C2C> &ax, const std::vector<int> &dims, int nd,
+                         long long outer, const void *in, void *buf, int dir, bool forward_scale_rule,
+                         double extra_scale, void *scratch, long long scratch_elems, hipStream_t s)
+{
+    const size_t cb = cx_bytes(p.prec);
+    const long long Pn = prod(dims, 0, nd);
+    auto scl = [&](int i) {
+        double v = forward_scale_rule ? 1.0 : axis_inv_scale(p.norm, dims[i]);
+        return v;
+    };
+    // last axis (contiguous), out of place
+    const int lastn = dims[nd - 1];
+    const long long lines = outer * Pn / lastn;
+    const C2C &el = ax[nd - 1];
+    hipError_t e;
+    double first_scale = scl(nd - 1) * extra_scale;
+    if (lastn == 1) {
+        if (in != buf) {
+            e = hipMemcpyAsync(buf, in, (size_t)outer * Pn * cb, hipMemcpyDeviceToDevice, s);
+            if (e != hipSuccess) return e;
+        }
+        if (first_scale != 1.0) {
+            e = launch_scale_real(buf, 2 * outer * Pn, p.prec, first_scale, s);
+            if (e != hipSuccess) return e;
+        }
+    } else {
+        const long long fit = el.scratch_per_line ? scratch_elems / el.scratch_per_line : 0;
+        e = el.run(in, buf, lines, lastn, lastn, dir, first_scale, scratch, fit, s);
+        if (e != hipSuccess) return e;
+    }
+    for (int i = nd - 2; i >= 0; i--) {
+        const long long A = outer * prod(dims, 0, i), B = prod(dims, i + 1, nd);
+        e = c2c_axis(ax[i], buf, A, B, dir, scl(i), scratch, scratch_elems, s);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// trig transform along axis i of a row-major (outer, dims...) real array.
+static hipError_t trig_axis(const nfft_plan &p, int i, int op, const void *in, void *out, double w0, void *scr,
+                            hipStream_t s)
+{
+    const size_t rb = re_bytes(p.prec);
+    const int n = p.dims[i];
+    const long long A = p.batch * prod(p.dims, 0, i), B = prod(p.dims, i + 1, p.d);
+    const long long tot = A * n * B;
+    void *full0 = scr, *full1 = (char *)scr + (size_t)tot * rb;
+    void *eng = (char *)scr + (size_t)(p.d > 1 || p.n_active > 1 ? 2 * tot : 0) * rb;
+    const long long eng_elems =
+        (long long)((p.work_bytes - (size_t)(p.d > 1 || p.n_active > 1 ? 2 * tot : 0) * rb) / cx_bytes(p.prec));
+    if (B == 1) return p.tax[i].run(op, in, out, A, n, n, 1.0, w0, eng, eng_elems, s);
+    hipError_t e = launch_transpose(in, full0, A, n, B, (int)rb, s);
+    if (e != hipSuccess) return e;
+    e = p.tax[i].run(op, full0, full1, A * B, n, n, 1.0, w0, eng, eng_elems, s);
+    if (e != hipSuccess) return e;
+    return launch_transpose(full1, out, A, B, n, (int)rb, s);
+}
+
+int nfft_plan::execute(int op, const void *in, void *out, hipStream_t s, int vjp_mode)
+{
+    const size_t cb = cx_bytes(prec), rb = re_bytes(prec);
+    hipError_t e = hipSuccess;
+    if (kind == NFFT_KIND_DFT) {
+        if (op != NFFT_OP_DFT && op != NFFT_OP_INVDFT) {
+            set_error("operation does not match a DFT plan");
+            return NFFT_E_INVALID;
+        }
+        int dir = op == NFFT_OP_DFT ? -1 : 1;
+        bool fwd_rule = op == NFFT_OP_DFT;
+        if (vjp_mode) {
+            // vjp(dft) = DFT^H (unnormalised); vjp(invdft) = DFT with invdft's scaling
+            if (op == NFFT_OP_DFT) { dir = 1; fwd_rule = true; }
+            else { dir = -1; fwd_rule = false; }
+        }
+        const long long eng_elems = (long long)(work_bytes / cb);
+        e = dft_nd(*this, ax, dims, d, batch, in, out, dir, fwd_rule, 1.0, work, eng_elems, s);
+    } else if (kind == NFFT_KIND_REALDFT) {
+        const int last = dims[d - 1];
+        const long long H = last / 2 + 1, Pl = P / last;
+        const long long lines = batch * Pl;
+        const bool forward = (op == NFFT_OP_REALDFT) != (vjp_mode != 0);
+        if (op != NFFT_OP_REALDFT && op != NFFT_OP_INVREALDFT) {
+            set_error("operation does not match a realdft plan");
+            return NFFT_E_INVALID;
+        }
+        if (in == out && batch > 1) {
+            set_error("in-place realdft/invrealdft needs batch == 1");
+            return NFFT_E_INVALID;
+        }
+        // scratch layout: [spectrum copy (batch*Pl*H)] [engine]
+        const long long copy = d > 1 ? batch * Pl * H : 0;
+        void *eng = (char *)work + (size_t)copy * cb;
+        const long long eng_elems = (long long)(work_bytes / cb) - copy;
+        // leading-axis complex dims of the half spectrum
+        std::vector<int> cd(dims.begin(), dims.end());
+        cd[d - 1] = (int)H;
+        if (forward) {
+            // op realdft, or the vjp of invrealdft (bins 0 < k < N/2 doubled)
+            const bool vjp_inv = op == NFFT_OP_INVREALDFT;
+            e = real.fwd(in, out, lines, last, H, 1.0, vjp_inv ? 1 : 0, eng, eng_elems, s);
+            if (e == hipSuccess && d > 1) {
+                for (int i = d - 2; i >= 0 && e == hipSuccess; i--) {
+                    const long long A = batch * prod(dims, 0, i), B = prod(cd, i + 1, d);
+                    const double sc = vjp_inv ? axis_inv_scale(norm, dims[i]) : 1.0;
+                    e = c2c_axis(ax[i], out, A, B, -1, sc, eng, eng_elems, s);
+                }
+            }
+        } else {
+            // op invrealdft, or the vjp of realdft (C^H on leading axes, then
+            // the half-weighted invrealdft)
+            const bool vjp_real = op == NFFT_OP_REALDFT;
+            const void *src = in;
+            if (d > 1) {
+                void *cp = work;
+                e = hipMemcpyAsync(cp, in, (size_t)batch * Pl * H * cb, hipMemcpyDeviceToDevice, s);
+                for (int i = d - 2; i >= 0 && e == hipSuccess; i--) {
+                    const long long A = batch * prod(dims, 0, i), B = prod(cd, i + 1, d);
+                    const double sc = vjp_real ? 1.0 : axis_inv_scale(norm, dims[i]);
+                    e = c2c_axis(ax[i], cp, A, B, 1, sc, eng, eng_elems, s);
+                }
+                src = cp;
+            }
+            if (e == hipSuccess) e = real.inv(src, out, lines, H, last, 1.0, vjp_real ? 1 : 0, eng, eng_elems, s);
+        }
+    } else {
+        const bool t4 = kind == NFFT_KIND_T4;
+        const bool ok_op = t4 ? (op == NFFT_OP_DCT4 || op == NFFT_OP_DST4)
+                              : (op >= NFFT_OP_DCT2 && op <= NFFT_OP_DST3);
+        if (!ok_op) {
+            set_error("operation does not match this trigonometric plan");
+            return NFFT_E_INVALID;
+        }
+        int run_op = op;
+        double w0 = 1.0;
+        if (vjp_mode && !t4) {
+            // (REDFT10)^T = REDFT01 diag(2,1,..,1); (RODFT10)^T = RODFT01 diag(1,..,1,2)
+            switch (op) {
+            case NFFT_OP_DCT2: run_op = NFFT_OP_DCT3; w0 = 2.0; break;
+            case NFFT_OP_DST2: run_op = NFFT_OP_DST3; w0 = 2.0; break;
+            case NFFT_OP_DCT3: run_op = NFFT_OP_DCT2; w0 = 0.5; break;
+            case NFFT_OP_DST3: run_op = NFFT_OP_DST2; w0 = 0.5; break;
+            }
+        }
+        if (n_active == 1 && d == 1) {
+            e = tax[0].run(run_op, in, out, batch, dims[0], dims[0], 1.0, w0, work,
+                           (long long)(work_bytes / cb), s);
+        } else {
+            // axes n_active-1 .. 0; the first reads `in`, the rest work in place on `out`
+            const void *src = in;
+            for (int i = n_active - 1; i >= 0 && e == hipSuccess; i--) {
+                e = trig_axis(*this, i, run_op, src, out, w0, work, s);
+                src = out;
+            }
+        }
+    }
+    if (e != hipSuccess) {
+        set_error(std::string("execute: ") + hipGetErrorString(e));
+        return NFFT_E_HIP;
+    }
+    (void)rb;
+    return NFFT_OK;
+}
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char *nfft_last_error(void) { return nfft::last_error(); }
+
+int nfft_plan_create(nfft_plan **plan, int kind, int prec, int d, const int *dims, long long batch, int norm)
+{
+    clear_error();
+    if (!plan) {
+        set_error("plan pointer is NULL");
+        return NFFT_E_INVALID;
+    }
+    *plan = nullptr;
+    if (kind < 0 || kind > 3 || (prec != NFFT_F32 && prec != NFFT_F64) || d < 1 || d > 16 || !dims || batch < 1 ||
+        norm < 0 || norm > 2) {
+        set_error("invalid plan arguments");
+        return NFFT_E_INVALID;
+    }
+    long long p = 1;
+    for (int i = 0; i < d; i++) {
+        if (dims[i] <= 0) {
+            set_error("dimensions must be positive");
+            return NFFT_E_SIZE;
+        }
+        p *= dims[i];
+        if (p > (1LL << 31)) {
+            set_error("transform size exceeds 2^31 elements");
+            return NFFT_E_UNSUPPORTED;
+        }
+    }
+    // size rules of the reference's constructors (src/07_legacy.h:1405-1634)
+    if (kind == NFFT_KIND_REALDFT && !is_pow2(dims[d - 1])) {
+        set_error("realdft needs a power-of-two last dimension");
+        return NFFT_E_SIZE;
+    }
+    if (kind == NFFT_KIND_T2T3 || kind == NFFT_KIND_T4) {
+        for (int i = 0; i < d; i++)
+            if (!is_pow2(dims[i])) {
+                set_error("trigonometric transforms need power-of-two dimensions");
+                return NFFT_E_SIZE;
+            }
+    }
+    nfft_plan *pl = new (std::nothrow) nfft_plan();
+    if (!pl) {
+        set_error("out of host memory");
+        return NFFT_E_ALLOC;
+    }
+    if (!pl->init(kind, prec, d, dims, batch, norm)) {
+        const std::string msg = last_error();
+        pl->release();
+        delete pl;
+        set_error(msg);
+        if (msg.find("limit") != std::string::npos) return NFFT_E_UNSUPPORTED;
+        if (msg.find("hipMalloc") != std::string::npos) return NFFT_E_ALLOC;
+        return NFFT_E_HIP;
+    }
+    *plan = pl;
+    return NFFT_OK;
+}
+
+int nfft_plan_destroy(nfft_plan *plan)
+{
+    if (!plan) return NFFT_OK;
+    plan->release();
+    delete plan;
+    return NFFT_OK;
+}
+
+int nfft_execute(nfft_plan *plan, int op, const void *in, void *out, void *stream)
+{
+    clear_error();
+    if (!plan || !in || !out) {
+        set_error("NULL plan or buffer");
+        return NFFT_E_INVALID;
+    }
+    return plan->execute(op, in, out, (hipStream_t)stream, 0);
+}
+
+int nfft_vjp(nfft_plan *plan, int op, const void *grad_out, void *grad_in, void *stream)
+{
+    clear_error();
+    if (!plan || !grad_out || !grad_in) {
+        set_error("NULL plan or buffer");
+        return NFFT_E_INVALID;
+    }
+    return plan->execute(op, grad_out, grad_in, (hipStream_t)stream, 1);
+}
+
+long long nfft_plan_workspace_bytes(const nfft_plan *plan) { return plan ? (long long)plan->work_bytes : 0; }
+
+const char *nfft_plan_describe(const nfft_plan *plan) { return plan ? plan->desc.c_str() : ""; }
+
+int nfft_fill_uniform(void *out, long long count, unsigned long long seed, unsigned long long start, int prec,
+                      void *stream)
+{
+    clear_error();
+    if (!out || count < 0 || (prec != NFFT_F32 && prec != NFFT_F64)) {
+        set_error("invalid fill arguments");
+        return NFFT_E_INVALID;
+    }
+    hipError_t e = launch_fill_uniform(out, count, seed, start, prec, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        set_error(hipGetErrorString(e));
+        return NFFT_E_HIP;
+    }
+    return NFFT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,44 @@
+"""Helpers for the GPU parity tests: run a plan of the extended C ABI on
+torch-allocated device buffers (torch is only the allocator here) and
+compare with the oracle."""
+import numpy as np
+
+import oracle_lib as ol
+
+
+def tol_for(prec, n):
+    lg = max(1.0, float(np.log2(max(2, n))))
+    return (1e-6 if prec == "f32" else 1e-14) * lg
+
+
+def np_dtype(prec):
+    return np.float32 if prec == "f32" else np.float64
+
+
+def to_dev(x, prec, dev):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(x).astype(np_dtype(prec)))).to(dev)
+
+
+def run_plan(pkg, dev, op, dims, x, batch=1, prec="f32", norm="reference", vjp=False, inplace=False):
+    import torch
+    nin, nout = ol.out_shape(op, dims)
+    if vjp:
+        nin, nout = nout, nin
+    xin = to_dev(x, prec, dev)
+    assert xin.numel() == nin * batch, (xin.numel(), nin, batch)
+    out = xin if inplace else torch.empty(nout * batch, dtype=xin.dtype, device=dev)
+    plan = pkg.Plan(ol.OP_KIND[op], prec, dims, batch, norm)
+    torch.cuda.synchronize()
+    if vjp:
+        plan.vjp(op, xin.data_ptr(), out.data_ptr())
+    else:
+        plan.execute(op, xin.data_ptr(), out.data_ptr())
+    torch.cuda.synchronize()
+    plan.close()
+    return out.cpu().numpy().astype(np.float64)
+
+
+def rounded(x, prec):
+    """The FP64 view of the values the device actually received."""
+    return np.asarray(x).astype(np_dtype(prec)).astype(np.float64)

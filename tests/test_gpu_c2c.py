@@ -1,0 +1,176 @@
+"""GPU parity of the complex transforms (single-pass lines, four-step,
+Bluestein) against the oracle, through the C ABI.
+
+Tolerances (rel-L2 against the FP64 oracle applied to the same FP32-rounded
+inputs): FP32 1e-6 * log2 N, FP64 1e-14 * log2 N.  Bluestein FP64 is judged
+against numpy.fft (pocketfft), because the reference's own chirp phase is not
+reduced mod 2N and carries an error growing with N; the distance to the
+reference's algorithm is bounded by its own test tolerance (1e-9,
+tests/test_notoriousfft.c:659) instead.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from gpu_util import rounded, run_plan, tol_for
+
+pytestmark = pytest.mark.gpu
+
+LINE32 = [1, 2, 4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 16384]
+LINE64 = [1, 2, 4, 8, 16, 64, 256, 1024, 4096, 8192]
+
+
+def cx(a):
+    return a[0::2] + 1j * a[1::2]
+
+
+def il(z):
+    z = np.asarray(z).reshape(-1)
+    out = np.empty(2 * z.size)
+    out[0::2], out[1::2] = z.real, z.imag
+    return out
+
+
+@pytest.mark.parametrize("prec,n", [("f32", n) for n in LINE32] + [("f64", n) for n in LINE64])
+def test_pow2_single_pass(pkg, dev, prec, n):
+    batch = 5
+    x = rounded(ol.uniform(1, 2 * n * batch), prec)
+    for op in ("dft", "invdft"):
+        got = run_plan(pkg, dev, op, [n], x, batch, prec)
+        ref = ol.run_oracle(op, [n], x, batch)
+        err = ol.rel_l2(got, ref)
+        print(f"{prec} {op} N={n} rel-L2 {err:.3e}")
+        assert err < tol_for(prec, n), (op, err)
+
+
+@pytest.mark.parametrize("prec,n,batch", [("f32", 1 << 15, 3), ("f32", 1 << 16, 2), ("f32", 1 << 18, 2),
+                                          ("f32", 1 << 20, 2), ("f64", 1 << 14, 3), ("f64", 1 << 17, 2),
+                                          ("f64", 1 << 20, 1)])
+def test_pow2_four_step(pkg, dev, prec, n, batch):
+    x = rounded(ol.uniform(2, 2 * n * batch), prec)
+    for op in ("dft", "invdft"):
+        got = run_plan(pkg, dev, op, [n], x, batch, prec)
+        ref = ol.run_oracle(op, [n], x, batch)
+        err = ol.rel_l2(got, ref)
+        print(f"{prec} {op} N={n} rel-L2 {err:.3e}")
+        assert err < tol_for(prec, n), (op, err)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_largest_pow2_properties(pkg, dev, prec):
+    # at the largest supported length: round trip and Parseval (size-free checks)
+    n = 1 << 22 if prec == "f32" else 1 << 21
+    x = rounded(ol.uniform(3, 2 * n), prec)
+    y = run_plan(pkg, dev, "dft", [n], x, 1, prec)
+    back = run_plan(pkg, dev, "invdft", [n], y, 1, prec)
+    tol = tol_for(prec, n)
+    assert ol.rel_l2(back / n, x) < 2 * tol
+    assert abs(np.dot(y, y) / (n * np.dot(x, x)) - 1) < 2 * tol
+    # spot-check a handful of bins against a direct sum
+    z = cx(x)
+    k = np.array([0, 1, 7, n // 3, n - 1])
+    direct = np.array([np.sum(z * np.exp(-2j * np.pi * ((kk * np.arange(n)) % n) / n)) for kk in k])
+    assert np.max(np.abs(cx(y)[k] - direct)) / np.sqrt(np.sum(np.abs(z) ** 2)) < 10 * tol
+
+
+BLUE = [3, 5, 6, 7, 10, 12, 15, 17, 20, 100, 127, 1000, 1009, 4097, 10007, 65521]
+
+
+@pytest.mark.parametrize("prec,n", [("f32", n) for n in BLUE] + [("f64", n) for n in BLUE])
+def test_bluestein(pkg, dev, prec, n):
+    batch = 3
+    x = rounded(ol.uniform(4, 2 * n * batch), prec)
+    hi = il(np.fft.fft(cx(x).reshape(batch, n), axis=1))
+    got = run_plan(pkg, dev, "dft", [n], x, batch, prec)
+    err = ol.rel_l2(got, hi)
+    ref = ol.run_oracle("dft", [n], x, batch)
+    dref = ol.rel_l2(got, ref)
+    print(f"{prec} bluestein N={n}: rel-L2 vs pocketfft {err:.3e}, vs reference algorithm {dref:.3e}, "
+          f"reference vs pocketfft {ol.rel_l2(ref, hi):.3e}")
+    assert err < tol_for(prec, n)
+    assert dref < (tol_for(prec, n) if prec == "f32" else 1e-9)
+    # inverse: normalised by 1/N for non-power-of-two lengths (reference behaviour)
+    goti = run_plan(pkg, dev, "invdft", [n], x, batch, prec)
+    hii = il(np.fft.ifft(cx(x).reshape(batch, n), axis=1))
+    assert ol.rel_l2(goti, hii) < tol_for(prec, n)
+
+
+@pytest.mark.parametrize("norm,expect", [("none", 1.0), ("inverse", None), ("reference", None)])
+def test_norm_flags(pkg, dev, norm, expect):
+    for n in (64, 100):
+        x = ol.uniform(5, 2 * n)
+        got = run_plan(pkg, dev, "invdft", [n], x, 1, "f64", norm=norm)
+        raw = il(np.fft.ifft(cx(x))) * n
+        if norm == "none":
+            want = raw
+        elif norm == "inverse":
+            want = raw / n
+        else:
+            want = raw if n == 64 else raw / n
+        assert ol.rel_l2(got, want) < 1e-13, (norm, n)
+
+
+def test_inplace_and_odd_batches(pkg, dev):
+    for n, batch in ((1024, 7), (1 << 16, 3), (17, 9), (1 << 20, 1)):
+        x = rounded(ol.uniform(6, 2 * n * batch), "f32")
+        got = run_plan(pkg, dev, "dft", [n], x, batch, "f32", inplace=True)
+        ref = ol.run_oracle("dft", [n], x, batch)
+        assert ol.rel_l2(got, ref) < tol_for("f32", n), (n, batch)
+
+
+def test_dropin_f64_host_pointers(pkg, dev):
+    """The reference API through libnotorious_fft_hip_f64.so on host memory:
+    the C1 configuration (N = 4096 dft then invdft, seed 0) and edge sizes."""
+    lib = pkg.dropin("f64")
+    for n in (1, 2, 8, 16, 4096, 100, 1009, 1 << 16):
+        a = lib.notorious_fft_mkaux_dft_1d(n)
+        assert a
+        x = ol.uniform(0, 2 * n)
+        x0 = x.copy()
+        y = np.zeros_like(x)
+        z = np.zeros_like(x)
+        lib.notorious_fft_dft(x.ctypes.data, y.ctypes.data, a)
+        lib.notorious_fft_invdft(y.ctypes.data, z.ctypes.data, a)
+        assert np.array_equal(x, x0)  # input untouched
+        ref = ol.run_oracle("dft", [n], x)
+        e1 = ol.rel_l2(y, ref)
+        scale = n if (n & (n - 1)) == 0 else 1
+        e2 = ol.rel_l2(z, scale * x)
+        print(f"dropin f64 N={n}: dft rel-L2 {e1:.3e}, round trip {e2:.3e}")
+        blue = (n & (n - 1)) != 0
+        assert e1 < (1e-9 if blue else tol_for("f64", n))
+        assert e2 < tol_for("f64", n) * (4 if blue else 1)
+        # x == y in place
+        w = x.copy()
+        lib.notorious_fft_dft(w.ctypes.data, w.ctypes.data, a)
+        assert np.array_equal(w, y)
+        lib.notorious_fft_free_aux(a)
+    assert not lib.notorious_fft_mkaux_dft_1d(0)
+    assert not lib.notorious_fft_mkaux_realdft_1d(12)
+    lib.notorious_fft_free_aux(None)
+
+
+def test_dropin_f64_device_pointers(pkg, dev):
+    import torch
+    lib = pkg.dropin("f64")
+    n = 2048
+    x = ol.uniform(0, 2 * n)
+    xd = torch.from_numpy(x).to(dev)
+    yd = torch.empty_like(xd)
+    a = lib.notorious_fft_mkaux_dft_1d(n)
+    torch.cuda.synchronize()
+    lib.notorious_fft_dft(xd.data_ptr(), yd.data_ptr(), a)
+    lib.notorious_fft_free_aux(a)
+    assert ol.rel_l2(yd.cpu().numpy(), ol.run_oracle("dft", [n], x)) < tol_for("f64", n)
+
+
+def test_dropin_f32(pkg, dev):
+    lib = pkg.dropin("f32")
+    for n in (16, 1024, 100):
+        a = lib.notorious_fft_mkaux_dft_1d(n)
+        x = ol.uniform(0, 2 * n).astype(np.float32)
+        y = np.zeros_like(x)
+        lib.notorious_fft_dft(x.ctypes.data, y.ctypes.data, a)
+        lib.notorious_fft_free_aux(a)
+        ref = ol.run_oracle("dft", [n], x.astype(np.float64))
+        assert ol.rel_l2(y.astype(np.float64), ref) < tol_for("f32", n)

@@ -1,0 +1,74 @@
+"""GPU parity of realdft / invrealdft and the DCT/DST family (types 2, 3, 4)
+against the oracle, single-pass and multi-pass lengths, FP32 and FP64.
+Tolerance: rel-L2 <= 1e-6 log2 N (FP32), 1e-14 log2 N (FP64)."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from gpu_util import rounded, run_plan, tol_for
+
+pytestmark = pytest.mark.gpu
+
+R32 = [1, 2, 4, 8, 16, 64, 256, 1024, 4096, 16384, 32768, 1 << 16, 1 << 17]
+R64 = [1, 2, 4, 16, 1024, 8192, 16384, 1 << 15]
+
+
+@pytest.mark.parametrize("prec,n", [("f32", n) for n in R32] + [("f64", n) for n in R64])
+def test_realdft(pkg, dev, prec, n):
+    batch = 3
+    x = rounded(ol.uniform(21, n * batch), prec)
+    got = run_plan(pkg, dev, "realdft", [n], x, batch, prec)
+    ref = ol.run_oracle("realdft", [n], x, batch)
+    err = ol.rel_l2(got, ref)
+    # inverse on an arbitrary half spectrum: the imaginary parts of bins 0 and
+    # N/2 are ignored, as the reference does (src/07_legacy.h:257-258)
+    h = n // 2 + 1
+    z = rounded(ol.uniform(22, 2 * h * batch), prec)
+    goti = run_plan(pkg, dev, "invrealdft", [n], z, batch, prec)
+    refi = ol.run_oracle("invrealdft", [n], z, batch)
+    erri = ol.rel_l2(goti, refi)
+    print(f"{prec} realdft N={n}: fwd {err:.3e} inv {erri:.3e}")
+    assert err < tol_for(prec, n)
+    assert erri < tol_for(prec, n)
+
+
+TRIG = ["dct2", "dst2", "dct3", "dst3", "dct4", "dst4"]
+T32 = [1, 2, 4, 8, 16, 128, 1024, 16384, 32768, 1 << 16]
+T64 = [1, 2, 4, 32, 1024, 16384, 1 << 15]
+
+
+@pytest.mark.parametrize("op", TRIG)
+@pytest.mark.parametrize("prec,n", [("f32", n) for n in T32] + [("f64", n) for n in T64])
+def test_trig(pkg, dev, op, prec, n):
+    batch = 3
+    x = rounded(ol.uniform(23, n * batch), prec)
+    got = run_plan(pkg, dev, op, [n], x, batch, prec)
+    ref = ol.run_oracle(op, [n], x, batch)
+    err = ol.rel_l2(got, ref)
+    print(f"{prec} {op} N={n}: rel-L2 {err:.3e}")
+    assert err < tol_for(prec, n)
+
+
+def test_dropin_real_and_trig_host(pkg, dev):
+    lib = pkg.dropin("f64")
+    for n in (1, 2, 4, 16, 1024):
+        a = lib.notorious_fft_mkaux_realdft_1d(n)
+        x = ol.uniform(24, n)
+        z = np.zeros(2 * (n // 2 + 1))
+        lib.notorious_fft_realdft(x.ctypes.data, z.ctypes.data, a)
+        assert ol.rel_l2(z, ol.run_oracle("realdft", [n], x)) < tol_for("f64", n)
+        y = np.zeros(n)
+        lib.notorious_fft_invrealdft(z.ctypes.data, y.ctypes.data, a)
+        assert ol.rel_l2(y, n * x) < tol_for("f64", n)
+        lib.notorious_fft_free_aux(a)
+        b = lib.notorious_fft_mkaux_t2t3_1d(n)
+        c = lib.notorious_fft_mkaux_t4_1d(n)
+        for op, aux in (("dct2", b), ("dst2", b), ("dct3", b), ("dst3", b), ("dct4", c), ("dst4", c)):
+            y = np.zeros(n)
+            getattr(lib, "notorious_fft_" + op)(x.ctypes.data, y.ctypes.data, aux)
+            assert ol.rel_l2(y, ol.run_oracle(op, [n], x)) < tol_for("f64", n), (op, n)
+            w = x.copy()  # in place
+            getattr(lib, "notorious_fft_" + op)(w.ctypes.data, w.ctypes.data, aux)
+            assert np.array_equal(w, y)
+        lib.notorious_fft_free_aux(b)
+        lib.notorious_fft_free_aux(c)
