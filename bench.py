@@ -1,0 +1,221 @@
+"""Benchmark of the batched 1D C2C hot path on one MI355X.
+
+Workload (the headline configuration): batched 1D complex FFT, FP32,
+N = 2^20, batch = 256 (2^28 complex elements, 2 GiB in + 2 GiB out),
+inputs i.i.d. uniform [-1, 1) from the seed-0 SplitMix64 stream, resident
+in HBM before the timed region.  A step is one forward transform of the
+whole batch.  value = 5 N log2 N * batch / t (GFLOP/s).
+
+Also printed in the same JSON line:
+  roofline      the dominant kernel's algorithmic bytes per launch divided by
+                its average launch time, timed live with HIP events by the
+                library's kernel timer on the launch stream (second timed pass)
+  cpu_baseline  the reference library compiled from its own header
+                (oracle/_ref, FP64, OpenMP over the batch, one aux per thread)
+                on a bounded sample of the same workload on this host
+  sweep         the rest of the batched C2C configuration (N = 2^10..2^20,
+                FP32 and FP64, 2^28 elements each) with rel-L2 vs the oracle
+  accuracy      rel-L2 of sampled output transforms vs the FP64 oracle
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-sweep] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "batched 1D C2C GFLOP/s (5NlogN) + %HBM BW, FP32 vs FP64 at stated rel-L2 err"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_config(pkg, torch, n, prec, total_elems, steps, warmup, profile=False, check=0):
+    """Time `steps` batched forward transforms of length n; returns a dict."""
+    dt = torch.float32 if prec == "f32" else torch.float64
+    batch = total_elems // n
+    dev = torch.device("cuda:0")
+    x = torch.empty(2 * n * batch, dtype=dt, device=dev)
+    y = torch.empty_like(x)
+    pkg.fill_uniform(x.data_ptr(), x.numel(), 0, 0, prec)
+    plan = pkg.Plan("dft", prec, [n], batch)
+    stream = torch.cuda.current_stream().cuda_stream
+    for _ in range(warmup):
+        plan.execute("dft", x.data_ptr(), y.data_ptr(), stream)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(steps):
+        plan.execute("dft", x.data_ptr(), y.data_ptr(), stream)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = e0.elapsed_time(e1)
+    res = {"n": n, "prec": prec, "batch": batch, "steps": steps, "wall_s": wall,
+           "ms_per_step": 1e3 * wall / steps, "event_ms_per_step": ev_ms / steps,
+           "plan": plan.describe()}
+    flops = 5.0 * n * math.log2(n) * batch
+    alg_bytes = 2.0 * n * batch * (8 if prec == "f32" else 16)
+    res["gflops"] = flops / (res["ms_per_step"] * 1e-3) / 1e9
+    res["alg_gbs"] = alg_bytes / (res["ms_per_step"] * 1e-3) / 1e9
+    res["hbm_frac"] = res["alg_gbs"] / HBM_PEAK_GBS
+    if profile:
+        pkg.prof_enable(True, capacity=max(4096, 64 * steps))
+        torch.cuda.synchronize()
+        for _ in range(steps):
+            plan.execute("dft", x.data_ptr(), y.data_ptr(), stream)
+        torch.cuda.synchronize()
+        rec = pkg.prof_read()
+        pkg.prof_enable(False)
+        res["kernels"] = {k: {"launches": v[0], "avg_us": 1e3 * v[1] / v[0], "total_ms": v[1],
+                              "bytes_per_launch": v[2] / v[0], "gbs": v[2] / (v[1] * 1e-3) / 1e9}
+                          for k, v in rec.items()}
+    if check:
+        import numpy as np
+        import oracle_lib as ol
+        idx = [0, batch - 1][:check]
+        errs = []
+        for b in idx:
+            xs = x[2 * n * b:2 * n * (b + 1)].cpu().numpy().astype(np.float64)
+            ys = y[2 * n * b:2 * n * (b + 1)].cpu().numpy().astype(np.float64)
+            errs.append(ol.rel_l2(ys, ol.run_oracle("dft", [n], xs)))
+        res["rel_l2"] = max(errs)
+        res["rel_l2_tol"] = (1e-6 if prec == "f32" else 1e-14) * math.log2(n)
+    plan.close()
+    del x, y
+    torch.cuda.empty_cache()
+    return res
+
+
+def cpu_baseline(n=1 << 20, budget_s=10.0):
+    """The reference compiled from its own header, FP64, OpenMP over the batch
+    (one aux per thread), on a bounded sample of the workload."""
+    import ctypes
+    import numpy as np
+    import oracle_lib as ol
+
+    cores = min(16, os.cpu_count() or 1)
+    flags = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
+    name = "libnfft_ref_fast64.so" if ("avx512f" in flags and "avx512dq" in flags) else "libnfft_ref_fast64_avx2.so"
+    kind = "reference"
+    try:
+        ref = ol.Ref(name)
+        run = lambda x, b: ref.run("dft", [n], x, batch=b, nthreads=cores)  # noqa: E731
+    except (FileNotFoundError, OSError):
+        kind, name = "port", "oracle"
+        run = lambda x, b: ol.run_oracle("dft", [n], x, batch=b, nthreads=cores)  # noqa: E731
+    x1 = ol.uniform(0, 2 * n * cores)
+    t0 = time.perf_counter()
+    run(x1, cores)
+    per_round = time.perf_counter() - t0
+    rounds = max(1, min(8, int(budget_s / max(per_round, 1e-3))))
+    batch = rounds * cores
+    x = ol.uniform(0, 2 * n * batch)
+    t0 = time.perf_counter()
+    run(x, batch)
+    t = time.perf_counter() - t0
+    gflops = 5.0 * n * math.log2(n) * batch / t / 1e9
+    cpu_model = ""
+    for line in flags.splitlines():
+        if line.startswith("model name"):
+            cpu_model = line.split(":", 1)[1].strip()
+            break
+    del ctypes
+    return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": cores, "kind": kind,
+            "sample": f"{batch} x N=2^20 FP64 C2C forward ({name}, OpenMP over the batch, "
+                      f"{t:.2f} s wall); host {cpu_model}; nproc {os.cpu_count()}",
+            "alg_gbs": round(2 * n * 16 * batch / t / 1e9, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--n", type=int, default=1 << 20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "batched 1D transforms shard by splitting the batch, with no exchange", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__ as ge
+
+    pkg = ge.load_package()
+    total = 1 << 28
+    log(f"headline: N={args.n} FP32 batch {total // args.n}, warmup {args.warmup}, steps {args.steps}")
+    head = run_config(pkg, torch, args.n, "f32", total, args.steps, args.warmup, profile=True, check=2)
+    log(f"  {head['ms_per_step']:.3f} ms/step, {head['gflops']:.1f} GFLOP/s, "
+        f"{head['alg_gbs']:.1f} GB/s ({100 * head['hbm_frac']:.1f}% HBM), rel-L2 {head['rel_l2']:.2e}")
+    kern = head["kernels"]
+    dom = max(kern, key=lambda k: kern[k]["total_ms"])
+    dk = kern[dom]
+    roofline = {"bound": "hbm", "achieved": round(dk["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dk["gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "avg_launch_us": round(dk["avg_us"], 2), "bytes_per_launch": dk["bytes_per_launch"],
+                "kernels": {k: {"avg_us": round(v["avg_us"], 2), "launches": v["launches"],
+                                "gbs": round(v["gbs"], 1)} for k, v in kern.items()}}
+
+    sweep = []
+    if not args.no_sweep:
+        for prec in ("f32", "f64"):
+            for lg in range(10, 21):
+                n = 1 << lg
+                r = run_config(pkg, torch, n, prec, total, 5, 2, check=1)
+                sweep.append({"n": n, "prec": prec, "gflops": round(r["gflops"], 1),
+                              "alg_gbs": round(r["alg_gbs"], 1), "hbm_frac": round(r["hbm_frac"], 4),
+                              "ms": round(r["ms_per_step"], 4), "rel_l2": float(f"{r['rel_l2']:.3e}"),
+                              "plan": r["plan"]})
+                log(f"  sweep {prec} N=2^{lg}: {r['gflops']:.1f} GFLOP/s {100 * r['hbm_frac']:.1f}% HBM "
+                    f"rel-L2 {r['rel_l2']:.2e} [{r['plan']}]")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline()
+            log(f"  cpu baseline: {cpu['value']} GFLOP/s on {cpu['cores']} threads ({cpu['kind']})")
+        except Exception as exc:  # reported, not fatal for the GPU figure
+            cpu = {"value": None, "unit": "GFLOP/s", "cores": 0, "kind": "reference",
+                   "sample": f"failed: {exc}"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(head["gflops"], 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (SplitMix64 uniform [-1,1), seed 0, generated on device)",
+        "config": {"workload": f"batched 1D C2C forward, FP32, N=2^{int(math.log2(args.n))} x "
+                               f"{total // args.n} transforms (2^28 complex elements)",
+                   "n": args.n, "batch": total // args.n, "plan": head["plan"]},
+        "hbm_frac": round(head["hbm_frac"], 4),
+        "alg_gbs": round(head["alg_gbs"], 1),
+        "accuracy": {"rel_l2": float(f"{head['rel_l2']:.3e}"), "tol": head["rel_l2_tol"],
+                     "vs": "FP64 oracle on the same FP32 inputs, 2 sampled transforms"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "sweep": sweep,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -89,6 +89,18 @@ long long nfft_plan_workspace_bytes(const nfft_plan *plan);
  * 1024x1024", "bluestein M=2048", ...) for diagnostics. */
 const char *nfft_plan_describe(const nfft_plan *plan);
 
+/* Kernel timer.  nfft_prof_enable(1, capacity) clears the records and
+ * brackets every subsequent kernel launch of the library with HIP events on
+ * the launch stream (up to `capacity` launches); (0, 0) stops recording.
+ * nfft_prof_count() gives the number of kernel families seen ("name<prec,L>");
+ * nfft_prof_get() returns one family's launch count, summed event time in
+ * milliseconds and summed algorithmic bytes (compulsory input + output
+ * traffic of each launch); it waits for the recorded events. */
+int nfft_prof_enable(int on, int capacity);
+int nfft_prof_count(void);
+int nfft_prof_get(int idx, char *name, int name_len, long long *launches, double *total_ms,
+                  double *bytes);
+
 /* Message for the last error on this thread ("" if none). */
 const char *nfft_last_error(void);
 

@@ -89,6 +89,13 @@ def core():
         lib.nfft_last_error.argtypes = []
         lib.nfft_fill_uniform.restype = _i
         lib.nfft_fill_uniform.argtypes = [_vp, _ll, ctypes.c_ulonglong, ctypes.c_ulonglong, _i, _vp]
+        lib.nfft_prof_enable.restype = _i
+        lib.nfft_prof_enable.argtypes = [_i, _i]
+        lib.nfft_prof_count.restype = _i
+        lib.nfft_prof_count.argtypes = []
+        lib.nfft_prof_get.restype = _i
+        lib.nfft_prof_get.argtypes = [_i, ctypes.c_char_p, _i, ctypes.POINTER(_ll),
+                                      ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         if hasattr(lib, "nfft_ragged_create"):
             lib.nfft_ragged_create.restype = _i
             lib.nfft_ragged_create.argtypes = [ctypes.POINTER(_vp), _i, _i, ctypes.POINTER(_i), _i]
@@ -183,3 +190,23 @@ class Plan:
 def fill_uniform(ptr, count, seed, start=0, prec="f32", stream=0):
     check(core().nfft_fill_uniform(ptr, count, seed, start, F32 if prec == "f32" else F64, stream),
           "fill_uniform")
+
+
+def prof_enable(on=True, capacity=8192):
+    """Start (clear) or stop the library's per-launch kernel timer."""
+    check(core().nfft_prof_enable(1 if on else 0, capacity), "prof_enable")
+
+
+def prof_read():
+    """{family: (launches, total_ms, algorithmic_bytes)} of the recorded launches."""
+    lib = core()
+    n = lib.nfft_prof_count()
+    if n >= 1000000:
+        raise NativeError("kernel timer capacity exceeded")
+    out = {}
+    buf = ctypes.create_string_buffer(256)
+    for i in range(n):
+        cnt, ms, by = _ll(), ctypes.c_double(), ctypes.c_double()
+        check(lib.nfft_prof_get(i, buf, 256, ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(by)), "prof_get")
+        out[buf.value.decode()] = (cnt.value, ms.value, by.value)
+    return out

@@ -69,6 +69,15 @@ int bluerow_lpb_rt(int prec, int L);
 int line_lpb_rt(int L);
 
 // ---------------------------------------------------------------------------
+// kernel timing (nfft_plan.cpp): while enabled, each launch is bracketed by
+// HIP events on its own stream and attributed to a kernel family name with
+// its algorithmic bytes (compulsory input + output traffic of that launch).
+// ---------------------------------------------------------------------------
+bool prof_on();
+int prof_begin(hipStream_t s);  // returns a slot or -1
+void prof_end(int slot, hipStream_t s, const char *family, int prec, int L, double bytes);
+
+// ---------------------------------------------------------------------------
 // launchers (nfft_k_*.hip); return hipSuccess or the launch error
 // ---------------------------------------------------------------------------
 hipError_t launch_line(int prec, int L, int dir, const LineArgs &a, hipStream_t s);

@@ -8,41 +8,65 @@ namespace nfft {
 // ---------------------------------------------------------------------------
 // precision dispatch
 // ---------------------------------------------------------------------------
+// Each dispatcher records its launch with the kernel timer when it is on.
+#define NFFT_TIMED(FAMILY, L, BYTES, CALL)                                                                    \
+    do {                                                                                                      \
+        const int slot_ = prof_on() ? prof_begin(s) : -1;                                                     \
+        hipError_t e_ = (CALL);                                                                               \
+        if (slot_ >= 0) prof_end(slot_, s, FAMILY, prec, L, (BYTES));                                         \
+        return e_;                                                                                            \
+    } while (0)
+
+static double cxb(int prec) { return prec == F32 ? 8.0 : 16.0; }
+
 hipError_t launch_line(int prec, int L, int dir, const LineArgs &a, hipStream_t s)
 {
-    return prec == F32 ? line_impl<float>(L, dir, a, s) : line_impl<double>(L, dir, a, s);
+    NFFT_TIMED("k_line", L, 2.0 * a.nlines * L * cxb(prec),
+               prec == F32 ? line_impl<float>(L, dir, a, s) : line_impl<double>(L, dir, a, s));
 }
 hipError_t launch_blue_line(int prec, int M, const BlueArgs &a, hipStream_t s)
 {
-    return prec == F32 ? blue_line_impl<float>(M, a, s) : blue_line_impl<double>(M, a, s);
+    NFFT_TIMED("k_blue_line", M, 2.0 * a.nlines * a.n * cxb(prec),
+               prec == F32 ? blue_line_impl<float>(M, a, s) : blue_line_impl<double>(M, a, s));
 }
 hipError_t launch_col(int prec, int L, int dir, int mode, const ColArgs &a, hipStream_t s)
 {
-    return prec == F32 ? col_impl<float>(L, dir, mode, a, s) : col_impl<double>(L, dir, mode, a, s);
+    static const char *names[4] = {"k_col_plain", "k_col_twiddle", "k_col_blue_in", "k_col_blue_out"};
+    const double tile = (double)a.ntrans * a.ncols * L * cxb(prec);
+    const double nvb = (double)a.ntrans * a.nv * cxb(prec);
+    const double bytes = mode == COL_BLUE_IN || mode == COL_BLUE_OUT ? tile + nvb : 2.0 * tile;
+    NFFT_TIMED(names[mode & 3], L, bytes,
+               prec == F32 ? col_impl<float>(L, dir, mode, a, s) : col_impl<double>(L, dir, mode, a, s));
 }
 hipError_t launch_rowT(int prec, int L, int dir, const RowTArgs &a, hipStream_t s)
 {
-    return prec == F32 ? rowT_impl<float>(L, dir, a, s) : rowT_impl<double>(L, dir, a, s);
+    NFFT_TIMED("k_rowT", L, 2.0 * a.ntrans * a.nrows * L * cxb(prec),
+               prec == F32 ? rowT_impl<float>(L, dir, a, s) : rowT_impl<double>(L, dir, a, s));
 }
 hipError_t launch_blue_row(int prec, int L, const BlueRowArgs &a, hipStream_t s)
 {
-    return prec == F32 ? blue_row_impl<float>(L, a, s) : blue_row_impl<double>(L, a, s);
+    NFFT_TIMED("k_blue_row", L, 2.0 * a.ntrans * a.nrows * L * cxb(prec),
+               prec == F32 ? blue_row_impl<float>(L, a, s) : blue_row_impl<double>(L, a, s));
 }
 hipError_t launch_r2c(int prec, int L, const RtArgs &a, hipStream_t s)
 {
-    return prec == F32 ? r2c_impl<float>(L, false, a, s) : r2c_impl<double>(L, false, a, s);
+    NFFT_TIMED("k_r2c_line", L, (double)a.nlines * (2.0 * L + 1) * cxb(prec),
+               prec == F32 ? r2c_impl<float>(L, false, a, s) : r2c_impl<double>(L, false, a, s));
 }
 hipError_t launch_c2r(int prec, int L, const RtArgs &a, hipStream_t s)
 {
-    return prec == F32 ? r2c_impl<float>(L, true, a, s) : r2c_impl<double>(L, true, a, s);
+    NFFT_TIMED("k_c2r_line", L, (double)a.nlines * (2.0 * L + 1) * cxb(prec),
+               prec == F32 ? r2c_impl<float>(L, true, a, s) : r2c_impl<double>(L, true, a, s));
 }
 hipError_t launch_t23(int prec, int L, int op, const RtArgs &a, hipStream_t s)
 {
-    return prec == F32 ? t23_impl<float>(L, op, a, s) : t23_impl<double>(L, op, a, s);
+    NFFT_TIMED("k_t23_line", L, 2.0 * a.nlines * L * cxb(prec),
+               prec == F32 ? t23_impl<float>(L, op, a, s) : t23_impl<double>(L, op, a, s));
 }
 hipError_t launch_t4(int prec, int L, int op, const RtArgs &a, hipStream_t s)
 {
-    return prec == F32 ? t4_impl<float>(L, op, a, s) : t4_impl<double>(L, op, a, s);
+    NFFT_TIMED("k_t4_line", L, 2.0 * a.nlines * L * cxb(prec),
+               prec == F32 ? t4_impl<float>(L, op, a, s) : t4_impl<double>(L, op, a, s));
 }
 
 template <int L> static int col_lpb_of(int prec) { return prec == F32 ? col_lpb<float, L>() : col_lpb<double, L>(); }
@@ -241,17 +265,23 @@ hipError_t launch_transpose(const void *in, void *out, long long batch, long lon
                             hipStream_t s)
 {
     if (batch <= 0 || rows <= 0 || cols <= 0) return hipSuccess;
-    if (batch > 65535 || (cols + 31) / 32 > 0x7fffffffLL || (rows + 31) / 32 > 65535) return hipErrorInvalidValue;
-    dim3 g((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32), (unsigned)batch), b(32, 8);
-    switch (esz) {
-    case 4: hipLaunchKernelGGL((k_transpose<float>), g, b, 0, s, (const float *)in, (float *)out, rows, cols); break;
-    case 8: hipLaunchKernelGGL((k_transpose<float2>), g, b, 0, s, (const float2 *)in, (float2 *)out, rows, cols); break;
-    case 16:
-        hipLaunchKernelGGL((k_transpose<double2>), g, b, 0, s, (const double2 *)in, (double2 *)out, rows, cols);
-        break;
-    default: return hipErrorInvalidValue;
+    if ((cols + 31) / 32 > 0x7fffffffLL || (rows + 31) / 32 > 65535) return hipErrorInvalidValue;
+    if (esz != 4 && esz != 8 && esz != 16) return hipErrorInvalidValue;
+    // grid.z <= 65535: walk the batch in slices
+    for (long long b0 = 0; b0 < batch; b0 += 65535) {
+        const long long nb = batch - b0 < 65535 ? batch - b0 : 65535;
+        const size_t off = (size_t)b0 * rows * cols * esz;
+        const void *src = (const char *)in + off;
+        void *dst = (char *)out + off;
+        dim3 g((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32), (unsigned)nb), b(32, 8);
+        if (esz == 4) hipLaunchKernelGGL((k_transpose<float>), g, b, 0, s, (const float *)src, (float *)dst, rows, cols);
+        else if (esz == 8)
+            hipLaunchKernelGGL((k_transpose<float2>), g, b, 0, s, (const float2 *)src, (float2 *)dst, rows, cols);
+        else hipLaunchKernelGGL((k_transpose<double2>), g, b, 0, s, (const double2 *)src, (double2 *)dst, rows, cols);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
     }
-    return hipGetLastError();
+    return hipSuccess;
 }
 
 __global__ void k_scale_real(void *buf, long long count, int prec, double sc)

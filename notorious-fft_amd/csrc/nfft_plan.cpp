@@ -48,6 +48,50 @@ static int ilog2ll(long long n)
 }
 
 // ---------------------------------------------------------------------------
+// kernel timer
+// ---------------------------------------------------------------------------
+namespace {
+struct ProfRec {
+    std::string family;
+    int prec, L;
+    double bytes;
+    int slot;
+};
+struct Prof {
+    std::mutex mu;
+    bool on = false;
+    std::vector<hipEvent_t> ev;  // 2 per slot
+    int used = 0;
+    bool overflow = false;
+    std::vector<ProfRec> recs;
+};
+Prof g_prof;
+}  // namespace
+
+bool prof_on() { return g_prof.on; }
+
+int prof_begin(hipStream_t s)
+{
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    if (!g_prof.on) return -1;
+    if (2 * (g_prof.used + 1) > (int)g_prof.ev.size()) {
+        g_prof.overflow = true;
+        return -1;
+    }
+    const int slot = g_prof.used++;
+    if (hipEventRecord(g_prof.ev[2 * slot], s) != hipSuccess) return -1;
+    return slot;
+}
+
+void prof_end(int slot, hipStream_t s, const char *family, int prec, int L, double bytes)
+{
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    if (slot < 0 || 2 * slot + 1 >= (int)g_prof.ev.size()) return;
+    (void)hipEventRecord(g_prof.ev[2 * slot + 1], s);
+    g_prof.recs.push_back(ProfRec{family, prec, L, bytes, slot});
+}
+
+// ---------------------------------------------------------------------------
 // per-device table cache
 // ---------------------------------------------------------------------------
 namespace {
@@ -992,6 +1036,69 @@ int nfft_vjp(nfft_plan *plan, int op, const void *grad_out, void *grad_in, void 
 long long nfft_plan_workspace_bytes(const nfft_plan *plan) { return plan ? (long long)plan->work_bytes : 0; }
 
 const char *nfft_plan_describe(const nfft_plan *plan) { return plan ? plan->desc.c_str() : ""; }
+
+int nfft_prof_enable(int on, int capacity)
+{
+    clear_error();
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    if (on) {
+        if (capacity < 1) capacity = 4096;
+        while ((int)g_prof.ev.size() < 2 * capacity) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) {
+                set_error("hipEventCreate failed");
+                return NFFT_E_HIP;
+            }
+            g_prof.ev.push_back(e);
+        }
+        g_prof.used = 0;
+        g_prof.overflow = false;
+        g_prof.recs.clear();
+    }
+    g_prof.on = on != 0;
+    return NFFT_OK;
+}
+
+int nfft_prof_count(void)
+{
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    std::map<std::string, int> fam;
+    for (auto &r : g_prof.recs) fam[r.family + "<" + (r.prec ? "f64," : "f32,") + std::to_string(r.L) + ">"] = 1;
+    return (int)fam.size() + (g_prof.overflow ? 1000000 : 0);
+}
+
+int nfft_prof_get(int idx, char *name, int name_len, long long *launches, double *total_ms, double *bytes)
+{
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    struct Agg {
+        long long n = 0;
+        double ms = 0, bytes = 0;
+    };
+    std::map<std::string, Agg> fam;
+    for (auto &r : g_prof.recs) {
+        float ms = 0;
+        if (hipEventSynchronize(g_prof.ev[2 * r.slot + 1]) != hipSuccess ||
+            hipEventElapsedTime(&ms, g_prof.ev[2 * r.slot], g_prof.ev[2 * r.slot + 1]) != hipSuccess) {
+            set_error("event timing failed");
+            return NFFT_E_HIP;
+        }
+        Agg &a = fam[r.family + "<" + (r.prec ? "f64," : "f32,") + std::to_string(r.L) + ">"];
+        a.n++;
+        a.ms += ms;
+        a.bytes += r.bytes;
+    }
+    int i = 0;
+    for (auto &kv : fam) {
+        if (i++ != idx) continue;
+        if (name && name_len > 0) snprintf(name, (size_t)name_len, "%s", kv.first.c_str());
+        if (launches) *launches = kv.second.n;
+        if (total_ms) *total_ms = kv.second.ms;
+        if (bytes) *bytes = kv.second.bytes;
+        return NFFT_OK;
+    }
+    set_error("no such profile entry");
+    return NFFT_E_INVALID;
+}
 
 int nfft_fill_uniform(void *out, long long count, unsigned long long seed, unsigned long long start, int prec,
                       void *stream)
