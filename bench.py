@@ -96,6 +96,94 @@ def run_config(pkg, torch, n, prec, total_elems, steps, warmup, profile=False, c
     return res
 
 
+def timed(torch, fn, steps=5, warmup=2):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def config_extras(pkg, torch):
+    """The other workloads of the configuration list, one timing each (FP32
+    unless stated): R2C batches, 3D C2C 512^3, 2D R2C 4096^2, DCT-2/DST-2/DCT-4
+    rows, the ragged batch of 10,000 mixed sizes (FP32 and FP64) and the
+    adjoint pair (forward + vjp) of the 2D C2C and the DCT-2."""
+    import ctypes
+    from ragged_sizes import ragged_sizes
+
+    dev = torch.device("cuda:0")
+    out = {}
+
+    def plan_run(name, kind, prec, dims, batch, op, nin, nout, flops, vjp=False):
+        dt = torch.float32 if prec == "f32" else torch.float64
+        x = torch.empty(nin * batch, dtype=dt, device=dev)
+        y = torch.empty(nout * batch, dtype=dt, device=dev)
+        pkg.fill_uniform(x.data_ptr(), x.numel(), 0, 0, prec)
+        p = pkg.Plan(kind, prec, dims, batch)
+        fn = (lambda: p.vjp(op, x.data_ptr(), y.data_ptr())) if vjp else \
+            (lambda: p.execute(op, x.data_ptr(), y.data_ptr()))
+        t = timed(torch, fn)
+        eb = 4 if prec == "f32" else 8
+        out[name] = {"ms": round(1e3 * t, 4), "gflops": round(flops * batch / t / 1e9, 1),
+                     "alg_gbs": round((nin + nout) * batch * eb / t / 1e9, 1),
+                     "hbm_frac": round((nin + nout) * batch * eb / t / 1e9 / HBM_PEAK_GBS, 4),
+                     "plan": p.describe()}
+        log(f"  {name}: {out[name]}")
+        p.close()
+        del x, y
+        torch.cuda.empty_cache()
+
+    n = 1 << 16
+    plan_run("r2c_f32_N2^16_x4096", "realdft", "f32", [n], 4096, "realdft", n, n + 2,
+             2.5 * n * math.log2(n))
+    n = 1 << 20
+    plan_run("r2c_f32_N2^20_x256", "realdft", "f32", [n], 256, "realdft", n, n + 2, 2.5 * n * math.log2(n))
+    p3 = 512 ** 3
+    plan_run("c2c3d_f32_512^3", "dft", "f32", [512, 512, 512], 1, "dft", 2 * p3, 2 * p3, 5 * p3 * 27)
+    p2 = 4096 * 4096
+    plan_run("r2c2d_f32_4096^2", "realdft", "f32", [4096, 4096], 1, "realdft", p2, 4096 * 2049 * 2,
+             2.5 * p2 * 24)
+    n = 1 << 14
+    for op in ("dct2", "dst2", "dct4"):
+        plan_run(f"{op}_f32_N2^14_x8192", "t2t3" if op != "dct4" else "t4", "f32", [n], 8192, op, n, n,
+                 2.5 * n * math.log2(n))
+    p2 = 2048 * 2048
+    plan_run("adj_c2c2d_f32_2048^2_x16_fwd", "dft", "f32", [2048, 2048], 16, "dft", 2 * p2, 2 * p2, 5 * p2 * 22)
+    plan_run("adj_c2c2d_f32_2048^2_x16_vjp", "dft", "f32", [2048, 2048], 16, "dft", 2 * p2, 2 * p2, 5 * p2 * 22,
+             vjp=True)
+    n = 1 << 14
+    plan_run("adj_dct2_f32_N2^14_x1024_vjp", "t2t3", "f32", [n], 1024, "dct2", n, n, 2.5 * n * math.log2(n),
+             vjp=True)
+    sizes = ragged_sizes(10000)
+    lib = pkg.core()
+    for prec in ("f32", "f64"):
+        h = ctypes.c_void_p()
+        arr = (ctypes.c_int * len(sizes))(*sizes)
+        pkg.check(lib.nfft_ragged_create(ctypes.byref(h), 0 if prec == "f32" else 1, len(sizes), arr, 0),
+                  "ragged_create")
+        total = sum(sizes)
+        dt = torch.float32 if prec == "f32" else torch.float64
+        x = torch.empty(2 * total, dtype=dt, device=dev)
+        y = torch.empty_like(x)
+        pkg.fill_uniform(x.data_ptr(), x.numel(), 0, 0, prec)
+        t = timed(torch, lambda: pkg.check(lib.nfft_ragged_execute(h, 0, x.data_ptr(), y.data_ptr(), None)))
+        flops = sum(5.0 * m * math.log2(m) for m in sizes)
+        eb = 8 if prec == "f32" else 16
+        out[f"ragged_{prec}_10000"] = {"ms": round(1e3 * t, 4), "gflops": round(flops / t / 1e9, 1),
+                                       "alg_gbs": round(2 * total * eb / t / 1e9, 1),
+                                       "hbm_frac": round(2 * total * eb / t / 1e9 / HBM_PEAK_GBS, 4),
+                                       "elements": total}
+        log(f"  ragged {prec}: {out[f'ragged_{prec}_10000']}")
+        lib.nfft_ragged_destroy(h)
+        del x, y
+        torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(n=1 << 20, budget_s=10.0):
     """The reference compiled from its own header, FP64, OpenMP over the batch
     (one aux per thread), on a bounded sample of the workload."""
@@ -181,6 +269,10 @@ def main():
                 log(f"  sweep {prec} N=2^{lg}: {r['gflops']:.1f} GFLOP/s {100 * r['hbm_frac']:.1f}% HBM "
                     f"rel-L2 {r['rel_l2']:.2e} [{r['plan']}]")
 
+    extras = {}
+    if not args.no_sweep:
+        extras = config_extras(pkg, torch)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -213,6 +305,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "sweep": sweep,
+        "configs": extras,
     }
     print(json.dumps(out), flush=True)
 

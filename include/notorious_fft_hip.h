@@ -7,6 +7,7 @@
  *   - device-resident plans ("device-aux"): tables and scratch in HBM;
  *   - uniform batches of contiguous transforms on a caller stream;
  *   - an explicit normalisation flag;
+ *   - ragged batches of mixed sizes (one persistent launch, graph-captured);
  *   - vector-Jacobian products (adjoints) of every transform;
  *   - status codes with a thread-local message.
  * Library: libnotorious_fft_hip.so (both precisions; the precision is a plan
@@ -23,6 +24,7 @@ extern "C" {
 #endif
 
 typedef struct nfft_plan nfft_plan;
+typedef struct nfft_ragged nfft_ragged;
 
 /* status codes */
 enum {
@@ -88,6 +90,28 @@ long long nfft_plan_workspace_bytes(const nfft_plan *plan);
 /* A short description of the execution strategy ("line", "four-step
  * 1024x1024", "bluestein M=2048", ...) for diagnostics. */
 const char *nfft_plan_describe(const nfft_plan *plan);
+
+/* Synchronises `stream` (a hipStream_t, NULL = default) and reports whether
+ * an earlier execute of `plan` hit a device-side fault: the persistent
+ * four-step kernel bounds each of its inter-workgroup dependency waits by a
+ * wall-clock timeout and raises a flag instead of spinning forever.  Returns
+ * NFFT_OK, or NFFT_E_HIP with the message set (the outputs of the affected
+ * executes are then undefined).  The flag is sticky for the plan's lifetime. */
+int nfft_plan_check(nfft_plan *plan, void *stream);
+
+/* Ragged batch: `count` 1D complex transforms of sizes[i] (1 <= N <= 65536,
+ * any N), packed back to back (transform i starts at element
+ * sizes[0] + ... + sizes[i-1]).  Execution is one persistent kernel launch
+ * (after a one-node counter reset), recorded into a HIP graph on first use
+ * for each (in, out, direction) and replayed afterwards.  Inverse transforms
+ * follow the reference's normalisation (1/N for non-power-of-two N only).
+ * flags bit 0: autotune the persistent grid (workgroups per CU) on a scratch
+ * batch; the choice is appended to the file named by NFFT_TUNE_CACHE, if set,
+ * and read back by later creates with the same size list. */
+int nfft_ragged_create(nfft_ragged **r, int prec, int count, const int *sizes, int flags);
+int nfft_ragged_execute(nfft_ragged *r, int inverse, const void *in, void *out, void *stream);
+int nfft_ragged_destroy(nfft_ragged *r);
+long long nfft_ragged_total(const nfft_ragged *r);
 
 /* Kernel timer.  nfft_prof_enable(1, capacity) clears the records and
  * brackets every subsequent kernel launch of the library with HIP events on

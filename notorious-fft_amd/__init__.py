@@ -85,6 +85,8 @@ def core():
         lib.nfft_plan_workspace_bytes.argtypes = [_vp]
         lib.nfft_plan_describe.restype = ctypes.c_char_p
         lib.nfft_plan_describe.argtypes = [_vp]
+        lib.nfft_plan_check.restype = ctypes.c_int
+        lib.nfft_plan_check.argtypes = [_vp, _vp]
         lib.nfft_last_error.restype = ctypes.c_char_p
         lib.nfft_last_error.argtypes = []
         lib.nfft_fill_uniform.restype = _i
@@ -170,6 +172,10 @@ class Plan:
 
     def describe(self):
         return core().nfft_plan_describe(self.handle).decode()
+
+    def check(self, stream=0):
+        """Synchronise `stream`; raise if a device-side wait of this plan timed out."""
+        check(core().nfft_plan_check(self.handle, stream), "plan_check")
 
     @property
     def workspace_bytes(self):

@@ -191,8 +191,9 @@ template <int L> struct Sched {
     static constexpr int R0 = L >> (4 * (S - 1));         // first radix
     static constexpr int radix(int s) { return s == 0 ? R0 : 16; }
     static constexpr int ns(int s) { return s == 0 ? 1 : R0 << (4 * (s - 1)); }
-    // twiddle table: stage s >= 1 holds ns(s) rows of 16 entries
-    static constexpr int twoff(int s) { return s <= 1 ? 0 : twoff(s - 1) + ns(s - 1) * 16; }
+    // twiddle table (FP64): stage s >= 1 holds ns(s) rows of the 4 base
+    // powers [w, w^2, w^4, w^8] of w = e^{-2 pi i k / (16 ns(s))}
+    static constexpr int twoff(int s) { return s <= 1 ? 0 : twoff(s - 1) + ns(s - 1) * 4; }
     static constexpr int TWN = S <= 1 ? 0 : twoff(S);
 };
 
@@ -219,35 +220,97 @@ template <int LPB> struct LayCol {
 };
 
 // ---------------------------------------------------------------------------
+// FP64 complex helpers for twiddle arithmetic
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 dmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T> __device__ __forceinline__ Cx<T> to_cx(double2 w)
+{
+    return cmk<Cx<T>>((T)w.x, (T)w.y);
+}
+// a * w (DIR < 0) or a * conj(w) (DIR > 0) with w in FP64: FP64 kernels
+// multiply in FP64, FP32 kernels round w once and multiply in FP32.
+template <int DIR, typename C> __device__ __forceinline__ C ctw64(C a, double2 w)
+{
+    return ctw<DIR>(a, cmk<C>((decltype(C::x))w.x, (decltype(C::x))w.y));
+}
+
+// Radix-16 stage twiddles w^r (r = 1..15) from the base powers
+// b = [w, w^2, w^4, w^8]: 11 FP64 products, each power at most 3 products
+// away from correctly rounded bases.
+template <int DIR, typename C> __device__ __forceinline__ void apply_stage_tw(C *a, const double2 (&b)[4])
+{
+    // ordered so that at most two FP64 temporaries are live at a time
+    a[1] = ctw64<DIR>(a[1], b[0]);
+    a[2] = ctw64<DIR>(a[2], b[1]);
+    a[4] = ctw64<DIR>(a[4], b[2]);
+    a[8] = ctw64<DIR>(a[8], b[3]);
+    a[9] = ctw64<DIR>(a[9], dmul(b[3], b[0]));
+    a[10] = ctw64<DIR>(a[10], dmul(b[3], b[1]));
+    a[12] = ctw64<DIR>(a[12], dmul(b[3], b[2]));
+    {
+        const double2 p3 = dmul(b[1], b[0]);
+        a[3] = ctw64<DIR>(a[3], p3);
+        a[11] = ctw64<DIR>(a[11], dmul(b[3], p3));
+        const double2 p7 = dmul(b[2], p3);
+        a[7] = ctw64<DIR>(a[7], p7);
+        a[15] = ctw64<DIR>(a[15], dmul(b[3], p7));
+    }
+    {
+        const double2 p5 = dmul(b[2], b[0]);
+        a[5] = ctw64<DIR>(a[5], p5);
+        a[13] = ctw64<DIR>(a[13], dmul(b[3], p5));
+    }
+    {
+        const double2 p6 = dmul(b[2], b[1]);
+        a[6] = ctw64<DIR>(a[6], p6);
+        a[14] = ctw64<DIR>(a[14], dmul(b[3], p6));
+    }
+}
+
+// ---------------------------------------------------------------------------
 // The line FFT.  All threads of the block must call it (it contains block
-// barriers when S > 1).  reg[q] <-> point j + NT*q.
+// barriers when S > 1).  reg[q] <-> point j + NT*q.  The base twiddles of
+// stage s+1 are requested at the start of stage s so that their latency
+// overlaps the butterflies and the LDS exchange.
 // ---------------------------------------------------------------------------
 template <typename T, int L, int DIR, class Lay>
 __device__ __forceinline__ void line_fft(Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds, const Lay &lay, int j,
-                                         const Cx<T> *__restrict__ tw)
+                                         const double2 *__restrict__ tw)
 {
     using S = Sched<L>;
     using C = Cx<T>;
+    double2 bw[4];  // base powers of the current / next twiddled stage
     sfor<0, S::S>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         constexpr int Rs = S::radix(s), Ns = S::ns(s), m = S::R / Rs;
+        if constexpr (s == 0 && S::S > 1) {
+            const double2 *t = tw + S::twoff(1) + (j & (S::ns(1) - 1)) * 4;
+#pragma unroll
+            for (int e = 0; e < 4; e++) bw[e] = t[e];
+        }
 #pragma unroll
         for (int i = 0; i < m; i++) {
-            const int v = j + S::NT * i;
             C a[Rs];
 #pragma unroll
             for (int r = 0; r < Rs; r++) a[r] = reg[i + r * m];
             if constexpr (s > 0) {
-                const int k = v & (Ns - 1);
-                const C *t = tw + S::twoff(s) + k * 16;
-#pragma unroll
-                for (int r = 1; r < Rs; r++) a[r] = ctw<DIR>(a[r], t[r]);
+                static_assert(Rs == 16 && m == 1, "twiddled stages are radix 16");
+                apply_stage_tw<DIR>(a, bw);
             }
             dftR<Rs, DIR>(a);
 #pragma unroll
             for (int r = 0; r < Rs; r++) reg[i + r * m] = a[r];
         }
         if constexpr (s + 1 < S::S) {
+            if constexpr (s > 0) {
+                // request the next stage's bases; the exchange hides the latency
+                const double2 *t = tw + S::twoff(s + 1) + (j & (S::ns(s + 1) - 1)) * 4;
+#pragma unroll
+                for (int e = 0; e < 4; e++) bw[e] = t[e];
+            }
 #pragma unroll
             for (int i = 0; i < m; i++) {
                 const int v = j + S::NT * i;
@@ -279,9 +342,29 @@ __device__ __forceinline__ double2 tw2_get(const Tw2 &t, uint32_t e)
     r.y = h.x * l.y + h.y * l.x;
     return r;
 }
-template <typename T> __device__ __forceinline__ Cx<T> to_cx(double2 w)
+// reg[q] *= w^{base_e + step_e * q} (conj for DIR > 0), q = 0..R-1: two
+// table lookups, then an FP64 recurrence (error ~R ulps of FP64).
+template <int DIR, typename C, int R>
+__device__ __forceinline__ void apply_tw2_series(C (&reg)[R], const Tw2 &t, uint32_t base_e, uint32_t step_e)
 {
-    return cmk<Cx<T>>((T)w.x, (T)w.y);
+    double2 w = tw2_get(t, base_e);
+    const double2 st = tw2_get(t, step_e);
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        reg[q] = ctw64<DIR>(reg[q], w);
+        if (q + 1 < R) w = dmul(w, st);
+    }
+}
+
+// Logical workgroup index for XCD-aware tiling: workgroups are dispatched
+// round-robin over the 8 XCDs (linear id i runs on XCD i % 8), so giving
+// each XCD a contiguous range of logical tiles keeps tiles that share
+// 128-byte lines (adjacent column groups) in the same L2.
+__device__ __forceinline__ long long xcd_logical(long long lin, long long total)
+{
+    const long long q = total >> 3, r = total & 7;
+    const long long x = lin & 7, k = lin >> 3;
+    return x * q + (x < r ? x : r) + k;
 }
 
 }  // namespace nfft

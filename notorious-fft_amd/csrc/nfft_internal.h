@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "nfft_kernels.h"
+#include "nfft_kernels_persist.h"
 #include "nfft_kernels_rt.h"
 
 namespace nfft {
@@ -90,12 +91,24 @@ hipError_t launch_c2r(int prec, int L, const RtArgs &a, hipStream_t s);
 hipError_t launch_t23(int prec, int L, int op, const RtArgs &a, hipStream_t s);
 hipError_t launch_t4(int prec, int L, int op, const RtArgs &a, hipStream_t s);
 
+// persistent four-step (nfft_k_persist.hip): shape `variant` (0 = default)
+// of a power-of-two length n; false if that length/variant is not built.
+struct PersistCfg {
+    int n1 = 0, n2 = 0, lpb1 = 0, lpb2 = 0;
+    int threads = 0;  // per workgroup
+    int per_cu = 0;   // resident workgroups per CU
+    int id = -1;
+};
+bool persist_find(int prec, long long n, int variant, PersistCfg *cfg);
+hipError_t launch_four_persist(const PersistCfg &c, int dir, const PersistArgs &a, int grid, hipStream_t s);
+hipError_t launch_persist_reset(unsigned *ctr, long long words, hipStream_t s);
+
 enum EwKind { EW_UNPACK = 0, EW_PACK = 1, EW_TRIG_PRE = 2, EW_TRIG_POST = 3, EW_N1 = 4 };
 hipError_t launch_ew(int prec, int kind, const EwArgs &a, long long nthreads_x, hipStream_t s);
 
 // table generation: out[i] = e^{-2 pi i m(i)/den} in FP64 (out64) and/or
 // rounded to FP32 (out32); mode 0: m = (a*i + c) mod den, 1: m = (i>>4)*(i&15),
-// 2: m = (i*i) mod den.
+// 2: m = (i*i) mod den, 3: m = (i>>2) << (i&3) (stage base powers).
 hipError_t launch_gen_phase(double2 *out64, float2 *out32, long long count, int mode, long long a,
                             long long c, long long den, hipStream_t s);
 // Bluestein filter h (length M): conj chirp wrapped (FP64)
@@ -140,6 +153,14 @@ struct C2C {
     const void *chirp = nullptr;
     void *H = nullptr;   // owned: Bluestein filter spectrum (scaled 1/m)
     long long scratch_per_line = 0;  // elements of Cx per line (ALG_FOUR: n, ALG_BLUE3: m)
+    int col_lpb = 0, row_lpb = 0;     // tile widths of the two passes (0: defaults)
+    // ALG_FOUR in one persistent launch (k_four_persist) when pctr is set
+    PersistCfg pcfg{};
+    unsigned *pctr = nullptr;         // owned: counter block for up to pcap transforms
+    long long pcap = 0;
+    int pgrid = 0;                    // workgroups of the persistent launch
+    long long ptimeout = 0;           // wall-clock ticks a dependency wait may take
+    int pgrab = 1;                    // queue items taken per access
 
     // Set up tables for length n; returns false with the error set.
     bool init(int n, int prec);
@@ -150,6 +171,9 @@ struct C2C {
     // scratch_per_line == 0).
     hipError_t run(const void *in, void *out, long long nlines, long long in_dist, long long out_dist, int dir,
                    double scale, void *scratch, long long scratch_lines, hipStream_t s) const;
+    // true if a persistent launch of this engine ever gave up a dependency
+    // wait (call after the stream has been synchronised)
+    bool device_fault() const;
 };
 
 // strided complex transform along one axis of a row-major array:

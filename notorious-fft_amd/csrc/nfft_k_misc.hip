@@ -150,6 +150,7 @@ __global__ void k_gen_phase(double2 *o64, float2 *o32, long long count, int mode
     unsigned long long m;
     if (mode == 0) m = ((unsigned long long)a * (unsigned long long)i + (unsigned long long)c) % ud;
     else if (mode == 1) m = ((unsigned long long)(i >> 4) * (unsigned long long)(i & 15)) % ud;
+    else if (mode == 3) m = ((unsigned long long)(i >> 2) << (i & 3)) % ud;
     else m = ((unsigned long long)i * (unsigned long long)i) % ud;
     double x = 2.0 * (double)m / (double)den;  // in [0, 2)
     if (x > 1.0) x -= 2.0;                     // in (-1, 1]

@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_line(LineArgs a
     C reg[S::R];
     tile_load<T, L, LPB>(reg, lds, src, a.in_dist, L, nrows, b, j);
     LayLine<T, L> lay{b * lpadlen<T, L>()};
-    line_fft<T, L, DIR>(reg, lds, lay, j, (const C *)a.tw);
+    line_fft<T, L, DIR>(reg, lds, lay, j, (const double2 *)a.tw);
     const T sc = (T)a.scale;
     if (sc != (T)1) {
 #pragma unroll
@@ -172,6 +172,7 @@ struct ColArgs {
     long long nv;              // Bluestein valid length
     int conj;                  // Bluestein: conjugate input / output
     double scale;
+    int lpb;                   // columns per workgroup (0: default for L)
 };
 
 template <int L, int LPB> __host__ __device__ constexpr int col_threads() { return Sched<L>::NT * LPB; }
@@ -184,8 +185,10 @@ __global__ void __launch_bounds__((col_threads<L, LPB>())) k_col(ColArgs a)
     constexpr int NLDS = S::S > 1 ? L * LPB : 1;
     __shared__ C lds[NLDS];
     const int b = threadIdx.x % LPB, j = threadIdx.x / LPB;
-    const int c = blockIdx.x * LPB + b;
-    const long long t = (long long)blockIdx.y + (long long)blockIdx.z * gridDim.y;
+    const long long lin = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    const long long lg = xcd_logical(lin, (long long)gridDim.x * gridDim.y * gridDim.z);
+    const int c = (int)(lg % gridDim.x) * LPB + b;
+    const long long t = lg / gridDim.x;
     const bool valid = c < a.ncols && t < a.ntrans;
     C reg[S::R];
     if constexpr (MODE == COL_BLUE_IN) {
@@ -208,13 +211,10 @@ __global__ void __launch_bounds__((col_threads<L, LPB>())) k_col(ColArgs a)
         for (int q = 0; q < S::R; q++) reg[q] = valid ? src[(long long)(j + S::NT * q) * a.S] : czero<T>();
     }
     LayCol<LPB> lay{b};
-    line_fft<T, L, DIR>(reg, lds, lay, j, (const C *)a.tw);
+    line_fft<T, L, DIR>(reg, lds, lay, j, (const double2 *)a.tw);
     if constexpr (MODE == COL_TWIDDLE || MODE == COL_BLUE_IN) {
-#pragma unroll
-        for (int q = 0; q < S::R; q++) {
-            const uint32_t e = (uint32_t)c * (uint32_t)(j + S::NT * q);
-            reg[q] = ctw<DIR>(reg[q], to_cx<T>(tw2_get(a.tw2, e)));
-        }
+        const uint32_t cc = valid ? (uint32_t)c : 0u;
+        apply_tw2_series<DIR>(reg, a.tw2, cc * (uint32_t)j, cc * (uint32_t)S::NT);
     }
     if (!valid) return;
     if constexpr (MODE == COL_BLUE_OUT) {
@@ -256,6 +256,7 @@ struct RowTArgs {
     long long ntrans;
     const void *tw;
     double scale;
+    int lpb;                      // rows per workgroup (0: default for L)
 };
 
 template <typename T, int L, int LPB> __host__ __device__ constexpr int rowT_lds()
@@ -271,15 +272,17 @@ __global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT(RowTArgs a)
     constexpr int NTH = S::NT * LPB;
     __shared__ C lds[rowT_lds<T, L, LPB>()];
     const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
-    const int r0 = blockIdx.x * LPB;
-    const long long t = (long long)blockIdx.y + (long long)blockIdx.z * gridDim.y;
+    const long long lin = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    const long long lg = xcd_logical(lin, (long long)gridDim.x * gridDim.y * gridDim.z);
+    const int r0 = (int)(lg % gridDim.x) * LPB;
+    const long long t = lg / gridDim.x;
     if (t >= a.ntrans) return;  // uniform per workgroup
     const int nrows = (a.nrows - r0) < LPB ? (a.nrows - r0) : LPB;
     const C *src = (const C *)a.in + t * a.in_dist + (long long)r0 * L;
     C reg[S::R];
     tile_load<T, L, LPB>(reg, lds, src, L, L, nrows, b, j);
     LayLine<T, L> lay{b * lpadlen<T, L>()};
-    line_fft<T, L, DIR>(reg, lds, lay, j, (const C *)a.tw);
+    line_fft<T, L, DIR>(reg, lds, lay, j, (const double2 *)a.tw);
     const T sc = (T)a.scale;
 #pragma unroll
     for (int q = 0; q < S::R; q++) {
@@ -340,10 +343,10 @@ __global__ void __launch_bounds__(Sched<M>::NT *line_lpb<M>()) k_blue_line(BlueA
         }
     }
     LayLine<T, M> lay{b * lpadlen<T, M>()};
-    line_fft<T, M, -1>(reg, lds, lay, j, (const C *)a.tw);
+    line_fft<T, M, -1>(reg, lds, lay, j, (const double2 *)a.tw);
 #pragma unroll
     for (int q = 0; q < S::R; q++) reg[q] = cmul(reg[q], H[j + S::NT * q]);
-    line_fft<T, M, 1>(reg, lds, lay, j, (const C *)a.tw);
+    line_fft<T, M, 1>(reg, lds, lay, j, (const double2 *)a.tw);
     const T sc = (T)a.scale;
 #pragma unroll
     for (int q = 0; q < S::R; q++) {
@@ -386,19 +389,16 @@ __global__ void __launch_bounds__(Sched<L>::NT *LPB) k_blue_row(BlueRowArgs a)
     C reg[S::R];
     tile_load<T, L, LPB>(reg, lds, buf, L, L, nrows, b, j);
     LayLine<T, L> lay{b * lpadlen<T, L>()};
-    line_fft<T, L, -1>(reg, lds, lay, j, (const C *)a.tw);
+    line_fft<T, L, -1>(reg, lds, lay, j, (const double2 *)a.tw);
     const int k1 = r0 + (b < nrows ? b : 0);
     const C *Hp = (const C *)a.Hp + (long long)k1 * L;
 #pragma unroll
     for (int q = 0; q < S::R; q++) reg[q] = cmul(reg[q], Hp[j + S::NT * q]);
-    line_fft<T, L, 1>(reg, lds, lay, j, (const C *)a.tw);
-#pragma unroll
-    for (int q = 0; q < S::R; q++) {
-        const uint32_t e = (uint32_t)k1 * (uint32_t)(j + S::NT * q);
-        reg[q] = cmulc(reg[q], to_cx<T>(tw2_get(a.tw2, e)));
-    }
+    line_fft<T, L, 1>(reg, lds, lay, j, (const double2 *)a.tw);
+    apply_tw2_series<1>(reg, a.tw2, (uint32_t)k1 * (uint32_t)j, (uint32_t)k1 * (uint32_t)S::NT);
     if constexpr (S::NT < 64) __syncthreads();
     tile_store<T, L, LPB>(reg, lds, buf, L, L, nrows, b, j);
 }
 
 }  // namespace nfft
+

@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_r2c_line(RtArgs
     C reg[S::R];
     tile_load<T, L, LPB>(reg, lds, src, a.in_dist / 2, L, nrows, b, j);
     LayLine<T, L> lay{b * lpadlen<T, L>()};
-    line_fft<T, L, -1>(reg, lds, lay, j, (const C *)a.tw);
+    line_fft<T, L, -1>(reg, lds, lay, j, (const double2 *)a.tw);
     C *line = lds + b * lpadlen<T, L>();
 #pragma unroll
     for (int q = 0; q < S::R; q++) line[lpad<T>(j + S::NT * q)] = reg[q];
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_c2r_line(RtArgs
         reg[q] = pack_pair(xk, xm, w[k]);
     }
     LayLine<T, L> lay{b * lpadlen<T, L>()};
-    line_fft<T, L, 1>(reg, lds, lay, j, (const C *)a.tw);
+    line_fft<T, L, 1>(reg, lds, lay, j, (const double2 *)a.tw);
     const T sc = (T)a.scale;
     if (sc != (T)1) {
 #pragma unroll
@@ -173,7 +173,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs
             reg[q] = cmk<C>(xv(2 * m), xv(2 * m + 1));
         }
         __syncthreads();
-        line_fft<T, L, -1>(reg, lds, lay, j, (const C *)a.tw);
+        line_fft<T, L, -1>(reg, lds, lay, j, (const double2 *)a.tw);
         C *ln = lds + b * lpadlen<T, L>();
 #pragma unroll
         for (int q = 0; q < S::R; q++) ln[lpad<T>(j + S::NT * q)] = reg[q];
@@ -257,7 +257,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs
             reg[q] = pack_pair(X(k), X(L - k), rw[k]);
         }
         __syncthreads();
-        line_fft<T, L, 1>(reg, lds, lay, j, (const C *)a.tw);
+        line_fft<T, L, 1>(reg, lds, lay, j, (const double2 *)a.tw);
         C *ln = lds + b * lpadlen<T, L>();
 #pragma unroll
         for (int q = 0; q < S::R; q++) ln[lpad<T>(j + S::NT * q)] = reg[q];
@@ -306,7 +306,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t4_line(RtArgs 
     }
     __syncthreads();
     LayLine<T, L> lay{b * lpadlen<T, L>()};
-    line_fft<T, L, -1>(reg, lds, lay, j, (const C *)a.tw);
+    line_fft<T, L, -1>(reg, lds, lay, j, (const double2 *)a.tw);
     C *ln = lds + b * lpadlen<T, L>();
 #pragma unroll
     for (int q = 0; q < S::R; q++) ln[lpad<T>(j + S::NT * q)] = reg[q];

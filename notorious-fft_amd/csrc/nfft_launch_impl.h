@@ -72,12 +72,35 @@ template <typename T> hipError_t blue_line_impl(int M, const BlueArgs &a, hipStr
 }
 
 // ---------------------------------------------------------------------------
-template <typename T, int L> constexpr bool col_fits()
+template <typename T, int L, int LPB> constexpr bool col_fits_v()
 {
-    constexpr int lpb = col_lpb<T, L>();
-    constexpr size_t lds = Sched<L>::S > 1 ? (size_t)L * lpb * sizeof(Cx<T>) : 0;
-    return L >= 2 && L <= 4096 && lds <= kLdsLimit && Sched<L>::NT * lpb <= 1024;
+    constexpr size_t lds = Sched<L>::S > 1 ? (size_t)L * LPB * sizeof(Cx<T>) : 0;
+    return L >= 2 && L <= 4096 && lds <= kLdsLimit && Sched<L>::NT * LPB <= 1024 && Sched<L>::NT * LPB >= 64;
 }
+template <typename T, int L> constexpr bool col_fits() { return col_fits_v<T, L, col_lpb<T, L>()>(); }
+
+template <typename T, int L, int LPB>
+hipError_t col_launch(int dir, int mode, const ColArgs &a, hipStream_t s)
+{
+    if constexpr (col_fits_v<T, L, LPB>()) {
+        unsigned gy, gz;
+        grid_trans(a.ntrans, gy, gz);
+        dim3 g((unsigned)((a.ncols + LPB - 1) / LPB), gy, gz), b(Sched<L>::NT * LPB);
+        if (mode == COL_PLAIN && dir < 0) hipLaunchKernelGGL((k_col<T, L, LPB, -1, COL_PLAIN>), g, b, 0, s, a);
+        else if (mode == COL_PLAIN) hipLaunchKernelGGL((k_col<T, L, LPB, 1, COL_PLAIN>), g, b, 0, s, a);
+        else if (mode == COL_TWIDDLE && dir < 0) hipLaunchKernelGGL((k_col<T, L, LPB, -1, COL_TWIDDLE>), g, b, 0, s, a);
+        else if (mode == COL_TWIDDLE) hipLaunchKernelGGL((k_col<T, L, LPB, 1, COL_TWIDDLE>), g, b, 0, s, a);
+        else if (mode == COL_BLUE_IN) hipLaunchKernelGGL((k_col<T, L, LPB, -1, COL_BLUE_IN>), g, b, 0, s, a);
+        else if (mode == COL_BLUE_OUT) hipLaunchKernelGGL((k_col<T, L, LPB, 1, COL_BLUE_OUT>), g, b, 0, s, a);
+        else return hipErrorInvalidValue;
+        return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+}
+
+// Tile widths selectable at run time for the lengths the four-step passes
+// use; other lengths always take the default width.
+template <int L> constexpr bool tunable_len() { return L >= 128 && L <= 2048; }
 
 template <typename T> hipError_t col_impl(int L, int dir, int mode, const ColArgs &a, hipStream_t s)
 {
@@ -85,19 +108,16 @@ template <typename T> hipError_t col_impl(int L, int dir, int mode, const ColArg
 #define NFFT_CASE(LL)                                                                                         \
     case LL:                                                                                                  \
         if constexpr (col_fits<T, LL>()) {                                                                    \
-            constexpr int lpb = col_lpb<T, LL>();                                                             \
-            unsigned gy, gz;                                                                                  \
-            grid_trans(a.ntrans, gy, gz);                                                                     \
-            dim3 g((unsigned)((a.ncols + lpb - 1) / lpb), gy, gz), b(Sched<LL>::NT * lpb);                    \
-            if (mode == COL_PLAIN && dir < 0) hipLaunchKernelGGL((k_col<T, LL, lpb, -1, COL_PLAIN>), g, b, 0, s, a); \
-            else if (mode == COL_PLAIN) hipLaunchKernelGGL((k_col<T, LL, lpb, 1, COL_PLAIN>), g, b, 0, s, a);     \
-            else if (mode == COL_TWIDDLE && dir < 0)                                                          \
-                hipLaunchKernelGGL((k_col<T, LL, lpb, -1, COL_TWIDDLE>), g, b, 0, s, a);                       \
-            else if (mode == COL_TWIDDLE) hipLaunchKernelGGL((k_col<T, LL, lpb, 1, COL_TWIDDLE>), g, b, 0, s, a); \
-            else if (mode == COL_BLUE_IN) hipLaunchKernelGGL((k_col<T, LL, lpb, -1, COL_BLUE_IN>), g, b, 0, s, a); \
-            else if (mode == COL_BLUE_OUT) hipLaunchKernelGGL((k_col<T, LL, lpb, 1, COL_BLUE_OUT>), g, b, 0, s, a); \
-            else return hipErrorInvalidValue;                                                                 \
-            return hipGetLastError();                                                                         \
+            if constexpr (tunable_len<LL>()) {                                                                \
+                switch (a.lpb) {                                                                              \
+                case 4: if constexpr (col_fits_v<T, LL, 4>()) return col_launch<T, LL, 4>(dir, mode, a, s); break;                                         \
+                case 8: if constexpr (col_fits_v<T, LL, 8>()) return col_launch<T, LL, 8>(dir, mode, a, s); break;                                         \
+                case 16: if constexpr (col_fits_v<T, LL, 16>()) return col_launch<T, LL, 16>(dir, mode, a, s); break;                                       \
+                case 32: if constexpr (col_fits_v<T, LL, 32>()) return col_launch<T, LL, 32>(dir, mode, a, s); break;                                       \
+                default: break;                                                                               \
+                }                                                                                             \
+            }                                                                                                 \
+            return col_launch<T, LL, col_lpb<T, LL>()>(dir, mode, a, s);                                      \
         }                                                                                                     \
         break;
         NFFT_POW2_LIST(NFFT_CASE)
@@ -106,11 +126,25 @@ template <typename T> hipError_t col_impl(int L, int dir, int mode, const ColArg
     return hipErrorInvalidValue;
 }
 
-template <typename T, int L> constexpr bool rowT_fits()
+template <typename T, int L, int LPB> constexpr bool rowT_fits_v()
 {
-    constexpr int lpb = rowT_lpb<T, L>();
     return L >= 2 && L <= (sizeof(T) == 4 ? 2048 : 1024) &&
-           (size_t)rowT_lds<T, L, lpb>() * sizeof(Cx<T>) <= kLdsLimit && Sched<L>::NT * lpb <= 1024;
+           (size_t)rowT_lds<T, L, LPB>() * sizeof(Cx<T>) <= kLdsLimit && Sched<L>::NT * LPB <= 1024 &&
+           Sched<L>::NT * LPB >= 64;
+}
+template <typename T, int L> constexpr bool rowT_fits() { return rowT_fits_v<T, L, rowT_lpb<T, L>()>(); }
+
+template <typename T, int L, int LPB> hipError_t rowT_launch(int dir, const RowTArgs &a, hipStream_t s)
+{
+    if constexpr (rowT_fits_v<T, L, LPB>()) {
+        unsigned gy, gz;
+        grid_trans(a.ntrans, gy, gz);
+        dim3 g((unsigned)((a.nrows + LPB - 1) / LPB), gy, gz), b(Sched<L>::NT * LPB);
+        if (dir < 0) hipLaunchKernelGGL((k_rowT<T, L, LPB, -1>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_rowT<T, L, LPB, 1>), g, b, 0, s, a);
+        return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
 }
 
 template <typename T> hipError_t rowT_impl(int L, int dir, const RowTArgs &a, hipStream_t s)
@@ -119,13 +153,16 @@ template <typename T> hipError_t rowT_impl(int L, int dir, const RowTArgs &a, hi
 #define NFFT_CASE(LL)                                                                                         \
     case LL:                                                                                                  \
         if constexpr (rowT_fits<T, LL>()) {                                                                   \
-            constexpr int lpb = rowT_lpb<T, LL>();                                                            \
-            unsigned gy, gz;                                                                                  \
-            grid_trans(a.ntrans, gy, gz);                                                                     \
-            dim3 g((unsigned)((a.nrows + lpb - 1) / lpb), gy, gz), b(Sched<LL>::NT * lpb);                    \
-            if (dir < 0) hipLaunchKernelGGL((k_rowT<T, LL, lpb, -1>), g, b, 0, s, a);                         \
-            else hipLaunchKernelGGL((k_rowT<T, LL, lpb, 1>), g, b, 0, s, a);                                  \
-            return hipGetLastError();                                                                         \
+            if constexpr (tunable_len<LL>()) {                                                                \
+                switch (a.lpb) {                                                                              \
+                case 4: if constexpr (rowT_fits_v<T, LL, 4>()) return rowT_launch<T, LL, 4>(dir, a, s); break;                                              \
+                case 8: if constexpr (rowT_fits_v<T, LL, 8>()) return rowT_launch<T, LL, 8>(dir, a, s); break;                                              \
+                case 16: if constexpr (rowT_fits_v<T, LL, 16>()) return rowT_launch<T, LL, 16>(dir, a, s); break;                                            \
+                case 32: if constexpr (rowT_fits_v<T, LL, 32>()) return rowT_launch<T, LL, 32>(dir, a, s); break;                                            \
+                default: break;                                                                               \
+                }                                                                                             \
+            }                                                                                                 \
+            return rowT_launch<T, LL, rowT_lpb<T, LL>()>(dir, a, s);                                          \
         }                                                                                                     \
         break;
         NFFT_POW2_LIST(NFFT_CASE)

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -147,7 +148,7 @@ struct StageInfo {
     int twoff(int s) const
     {
         int o = 0;
-        for (int t = 1; t < s; t++) o += ns(t) * 16;
+        for (int t = 1; t < s; t++) o += ns(t) * 4;
         return o;
     }
 };
@@ -161,16 +162,18 @@ static StageInfo stage_info(int L)
     return si;
 }
 
-const void *stage_table(int prec, int L)
+// Stage twiddles of an L-point line: FP64 base powers [w, w^2, w^4, w^8] of
+// w = e^{-2 pi i k/(16 Ns)} for every butterfly index k of every stage >= 1
+// (the same table serves both precisions).
+const void *stage_table(int /*prec*/, int L)
 {
     const StageInfo si = stage_info(L);
-    const std::string key = "stage/" + std::to_string(prec) + "/" + std::to_string(L);
-    return cache_get(key, (size_t)std::max(1, si.twn) * cx_bytes(prec), [&](void *p, hipStream_t st) {
+    const std::string key = "stage64/" + std::to_string(L);
+    return cache_get(key, (size_t)std::max(1, si.twn) * 16, [&](void *p, hipStream_t st) {
         for (int s = 1; s < si.S; s++) {
-            const long long cnt = (long long)si.ns(s) * 16;
+            const long long cnt = (long long)si.ns(s) * 4;
             const int off = si.twoff(s);
-            hipError_t e = launch_gen_phase(prec == F64 ? (double2 *)p + off : nullptr,
-                                            prec == F32 ? (float2 *)p + off : nullptr, cnt, 1, 0, 0, cnt, st);
+            hipError_t e = launch_gen_phase((double2 *)p + off, nullptr, cnt, 3, 0, 0, (long long)si.ns(s) * 16, st);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -224,6 +227,36 @@ static int env_int(const char *name, int dflt)
     return v && *v ? atoi(v) : dflt;
 }
 
+// Dependency waits of the persistent four-step give up after this long
+// (wall clock); a healthy wait lasts microseconds.
+static const double kPersistTimeoutSec = 2.0;
+// The persistent four-step is opt-in (NFFT_PERSIST=1) while the two-launch
+// path is faster on the measured sizes.
+static const int kPersistDefault = 0;
+// Transforms per persistent launch (bounds the counter block).
+static const long long kPersistCap = 1LL << 15;
+
+static bool persist_setup(C2C &c)
+{
+    int dev = 0, ncu = 0, khz = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
+        !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "device attribute") ||
+        !hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "device attribute"))
+        return false;
+    const size_t bytes = (size_t)persist_ctr_words(kPersistCap) * sizeof(unsigned);
+    if (!hip_ok(hipMalloc(&c.pctr, bytes), "hipMalloc")) return false;
+    if (!hip_ok(hipMemset(c.pctr, 0, bytes), "hipMemset")) {
+        (void)hipFree(c.pctr);
+        c.pctr = nullptr;
+        return false;
+    }
+    c.pcap = kPersistCap;
+    c.pgrid = std::max(1, ncu) * c.pcfg.per_cu;
+    c.ptimeout = (long long)(kPersistTimeoutSec * 1000.0 * std::max(khz, 1000));
+    c.pgrab = std::max(1, env_int("NFFT_PERSIST_GRAB", 1));
+    return true;
+}
+
 bool C2C::init(int n_, int prec_)
 {
     n = n_;
@@ -239,24 +272,31 @@ bool C2C::init(int n_, int prec_)
         return tw_a != nullptr;
     }
     if (is_pow2(n)) {
-        const int lg = ilog2ll(n);
-        int lg2 = std::min(lg / 2, ilog2ll(rowT_max(prec)));
-        const int forced = env_int("NFFT_FOUR_N2", 0);
-        if (forced > 0 && is_pow2(forced) && forced <= rowT_max(prec) && n / forced <= col_max(prec) &&
-            n / forced >= 2)
-            lg2 = ilog2ll(forced);
-        n2 = 1 << lg2;
-        n1 = n / n2;
-        if (n1 > col_max(prec)) {
-            set_error("power-of-two length " + std::to_string(n) + " exceeds this build's limit");
-            return false;
-        }
         algo = ALG_FOUR;
+        scratch_per_line = n;
+        if (env_int("NFFT_PERSIST", kPersistDefault) && persist_find(prec, n, env_int("NFFT_PERSIST_VAR", 0), &pcfg)) {
+            n1 = pcfg.n1;
+            n2 = pcfg.n2;
+            if (!persist_setup(*this)) return false;
+        } else {
+            const int lg = ilog2ll(n);
+            int lg2 = std::min(lg / 2, ilog2ll(rowT_max(prec)));
+            const int forced = env_int("NFFT_FOUR_N2", 0);
+            if (forced > 0 && is_pow2(forced) && forced <= rowT_max(prec) && n / forced <= col_max(prec) &&
+                n / forced >= 2)
+                lg2 = ilog2ll(forced);
+            n2 = 1 << lg2;
+            n1 = n / n2;
+            if (n1 > col_max(prec)) {
+                set_error("power-of-two length " + std::to_string(n) + " exceeds this build's limit");
+                return false;
+            }
+            col_lpb = env_int("NFFT_COL_LPB", 0);
+            row_lpb = env_int("NFFT_ROWT_LPB", 0);
+        }
         tw_a = stage_table(prec, n1);
         tw_b = stage_table(prec, n2);
-        if (!tw_a || !tw_b || !tw2_table(n, &tw2)) return false;
-        scratch_per_line = n;
-        return true;
+        return tw_a && tw_b && tw2_table(n, &tw2);
     }
     // Bluestein
     m = 1;
@@ -312,6 +352,16 @@ void C2C::release()
 {
     if (H) (void)hipFree(H);
     H = nullptr;
+    if (pctr) (void)hipFree(pctr);
+    pctr = nullptr;
+}
+
+bool C2C::device_fault() const
+{
+    if (!pctr) return false;
+    unsigned w[2] = {0, 0};
+    if (hipMemcpy(w, pctr + 1, sizeof w, hipMemcpyDeviceToHost) != hipSuccess) return true;
+    return (w[0] | w[1]) != 0;
 }
 
 std::string C2C::describe() const
@@ -319,7 +369,13 @@ std::string C2C::describe() const
     char b[128];
     switch (algo) {
     case ALG_LINE: snprintf(b, sizeof b, "line %d", n); break;
-    case ALG_FOUR: snprintf(b, sizeof b, "four-step %dx%d", n1, n2); break;
+    case ALG_FOUR:
+        if (pctr)
+            snprintf(b, sizeof b, "four-step %dx%d persistent (tiles %d/%d, %d workgroups)", n1, n2, pcfg.lpb1,
+                     pcfg.lpb2, pgrid);
+        else
+            snprintf(b, sizeof b, "four-step %dx%d", n1, n2);
+        break;
     case ALG_BLUE1: snprintf(b, sizeof b, "bluestein N=%d M=%d single-pass", n, m); break;
     case ALG_BLUE3: snprintf(b, sizeof b, "bluestein N=%d M=%d (%dx%d)", n, m, m1, m2); break;
     default: snprintf(b, sizeof b, "n=%d", n);
@@ -343,6 +399,34 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
     }
     case ALG_FOUR: {
         if (scratch_lines < 1 || !scratch) return hipErrorInvalidValue;
+        if (pctr) {
+            // one persistent launch per kPersistCap transforms; the scratch
+            // holds the rotating intermediate slots
+            for (long long l0 = 0; l0 < nlines; l0 += pcap) {
+                const long long cnt = std::min(pcap, nlines - l0);
+                PersistArgs a{};
+                a.in = (const char *)in + (size_t)l0 * in_dist * cb;
+                a.out = (char *)out + (size_t)l0 * out_dist * cb;
+                a.ntrans = cnt;
+                a.in_dist = in_dist;
+                a.out_dist = out_dist;
+                a.scratch = scratch;
+                a.slots = (int)std::min(std::min(scratch_lines, cnt), 1LL << 20);
+                a.ctr = pctr;
+                a.tw_a = tw_a;
+                a.tw_b = tw_b;
+                a.tw2 = tw2;
+                a.scale = scale;
+                a.timeout = ptimeout;
+                a.grab = pgrab;
+                const long long items = cnt * ((long long)n2 / pcfg.lpb1 + (long long)n1 / pcfg.lpb2);
+                const int grid = (int)std::min<long long>(pgrid, (items + pgrab - 1) / pgrab);
+                hipError_t e = launch_persist_reset(pctr, persist_ctr_words(cnt), s);
+                if (e == hipSuccess) e = launch_four_persist(pcfg, dir, a, grid, s);
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
+        }
         for (long long l0 = 0; l0 < nlines; l0 += scratch_lines) {
             const long long cnt = std::min(scratch_lines, nlines - l0);
             ColArgs c{};
@@ -356,9 +440,10 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
             c.tw = tw_a;
             c.tw2 = tw2;
             c.scale = 1.0;
+            c.lpb = col_lpb;
             hipError_t e = launch_col(prec, n1, dir, COL_TWIDDLE, c, s);
             if (e != hipSuccess) return e;
-            RowTArgs r{scratch, (char *)out + (size_t)l0 * out_dist * cb, n1, n, out_dist, cnt, tw_b, scale};
+            RowTArgs r{scratch, (char *)out + (size_t)l0 * out_dist * cb, n1, n, out_dist, cnt, tw_b, scale, row_lpb};
             e = launch_rowT(prec, n2, dir, r, s);
             if (e != hipSuccess) return e;
         }
@@ -663,6 +748,10 @@ static long long prod(const std::vector<int> &v, size_t b, size_t e)
 
 static const long long kChunkBytesDefault = 64LL << 20;
 
+// Lines of scratch an engine gets: one chunk of about NFFT_CHUNK_MB of
+// intermediate (default 64 MiB, well inside the 256 MB Infinity Cache).  The
+// chunked passes process that many lines per launch pair; the persistent
+// four-step uses them as its rotating slots.
 static long long chunk_lines(long long nlines, long long per_line_elems, size_t eb)
 {
     if (per_line_elems <= 0) return 0;
@@ -1036,6 +1125,27 @@ int nfft_vjp(nfft_plan *plan, int op, const void *grad_out, void *grad_in, void 
 long long nfft_plan_workspace_bytes(const nfft_plan *plan) { return plan ? (long long)plan->work_bytes : 0; }
 
 const char *nfft_plan_describe(const nfft_plan *plan) { return plan ? plan->desc.c_str() : ""; }
+
+int nfft_plan_check(nfft_plan *plan, void *stream)
+{
+    clear_error();
+    if (!plan) {
+        set_error("plan is NULL");
+        return NFFT_E_INVALID;
+    }
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess) {
+        set_error("hipStreamSynchronize failed");
+        return NFFT_E_HIP;
+    }
+    bool bad = plan->real.inner.device_fault();
+    for (const auto &e : plan->ax) bad = bad || e.device_fault();
+    for (const auto &t : plan->tax) bad = bad || t.cplx.device_fault() || t.real.inner.device_fault();
+    if (bad) {
+        set_error("persistent four-step: a dependency wait timed out");
+        return NFFT_E_HIP;
+    }
+    return NFFT_OK;
+}
 
 int nfft_prof_enable(int on, int capacity)
 {

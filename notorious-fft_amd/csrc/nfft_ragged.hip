@@ -1,1 +1,770 @@
-// ragged batches: see nfft_ragged.hip (added with the persistent kernel)
+// nfft_ragged.hip -- ragged batches of mixed-size 1D complex transforms in
+// one persistent kernel launch, recorded into a HIP graph.
+//
+// The reference has no batch API at all (src/07_legacy.h:29-57); a caller
+// with many transforms of different sizes loops over plans on the CPU.  Here
+// the whole ragged batch is one launch:
+//   - the host sorts the transforms into work items: a group of up to LPB
+//     same-size short transforms (power of two <= 4096, or Bluestein with
+//     M <= 4096) processed as LDS-resident lines, or one long transform
+//     (power of two up to 65536: four-step; Bluestein up to M = 131072:
+//     three passes) processed by a single workgroup through its own scratch
+//     slice, so no work item ever waits for another workgroup;
+//   - items are ordered longest-first and handed out by an atomic counter;
+//   - the graph holds [counter reset kernel, persistent kernel]; there is no
+//     memset node.
+// Inverse transforms follow the reference's normalisation: unnormalised for
+// powers of two, 1/N for Bluestein lengths (src/05_algorithms.h:315-333).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/notorious_fft_hip.h"
+#include "nfft_internal.h"
+
+namespace nfft {
+
+constexpr int RG_THREADS = 256;
+enum RagType { RG_LINE = 0, RG_BLUE1 = 1, RG_FOUR = 2, RG_BLUE3 = 3 };
+
+struct RagItem {
+    int type;
+    int L, L2;   // LINE: n; BLUE1: M; FOUR: n1 x n2; BLUE3: m1 x m2
+    int n;
+    int count;   // transforms in this item
+    int first;   // first entry in the id list
+    int tab;     // table descriptor
+    int pad_;
+};
+
+struct RagTab {
+    const void *tw_a, *tw_b;
+    const void *chirp, *H;
+    Tw2 tw2;
+};
+
+struct RagArgs {
+    const void *in;
+    void *out;
+    const RagItem *items;
+    int nitems;
+    const int *ids;
+    const long long *offs;
+    const RagTab *tabs;
+    int *counter;
+    void *scratch;
+    long long scratch_per_block;
+};
+
+template <int L> constexpr int rg_lines() { return RG_THREADS / Sched<L>::NT; }
+
+// ---------------------------------------------------------------------------
+// groups of short lines
+// ---------------------------------------------------------------------------
+template <typename T, int L, int DIR, bool BLUE>
+__device__ void rg_line(const RagArgs &a, const RagItem &it, Cx<T> *lds, long long *bases)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    constexpr int LPB = rg_lines<L>();
+    constexpr int PADL = lpadlen<T, L>();
+    const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
+    const RagTab tb = a.tabs[it.tab];
+    const int len = BLUE ? it.n : L;
+    if (threadIdx.x < LPB) bases[threadIdx.x] = threadIdx.x < it.count ? a.offs[a.ids[it.first + threadIdx.x]] : 0;
+    __syncthreads();
+    const C *in = (const C *)a.in;
+    C *out = (C *)a.out;
+    C reg[S::R];
+    if constexpr (S::NT >= 64) {
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const int p = j + S::NT * q;
+            reg[q] = (b < it.count && p < len) ? in[bases[b] + p] : czero<T>();
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < LPB * L; idx += RG_THREADS) {
+            const int bb = idx / L, p = idx & (L - 1);
+            lds[bb * PADL + lpad<T>(p)] = (bb < it.count && p < len) ? in[bases[bb] + p] : czero<T>();
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < S::R; q++) reg[q] = lds[b * PADL + lpad<T>(j + S::NT * q)];
+        __syncthreads();
+    }
+    LayLine<T, L> lay{b * PADL};
+    T scale = (T)1;
+    if constexpr (BLUE) {
+        const C *ch = (const C *)tb.chirp;
+        const C *H = (const C *)tb.H;
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const int p = j + S::NT * q;
+            if (p < len) reg[q] = cmul(DIR > 0 ? cconj(reg[q]) : reg[q], ch[p]);
+        }
+        line_fft<T, L, -1>(reg, lds, lay, j, (const double2 *)tb.tw_a);
+#pragma unroll
+        for (int q = 0; q < S::R; q++) reg[q] = cmul(reg[q], H[j + S::NT * q]);
+        line_fft<T, L, 1>(reg, lds, lay, j, (const double2 *)tb.tw_a);
+        if (DIR > 0) scale = (T)1 / (T)len;
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const int p = j + S::NT * q;
+            C v = p < len ? cmul(reg[q], ch[p]) : czero<T>();
+            if (DIR > 0) v = cconj(v);
+            reg[q] = cscale(v, scale);
+        }
+    } else {
+        line_fft<T, L, DIR>(reg, lds, lay, j, (const double2 *)tb.tw_a);
+    }
+    if constexpr (S::NT >= 64) {
+        if (b < it.count) {
+#pragma unroll
+            for (int q = 0; q < S::R; q++) {
+                const int p = j + S::NT * q;
+                if (p < len) out[bases[b] + p] = reg[q];
+            }
+        }
+    } else {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < S::R; q++) lds[b * PADL + lpad<T>(j + S::NT * q)] = reg[q];
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < LPB * L; idx += RG_THREADS) {
+            const int bb = idx / L, p = idx & (L - 1);
+            if (bb < it.count && p < len) out[bases[bb] + p] = lds[bb * PADL + lpad<T>(p)];
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// one long transform per workgroup
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void rg_fence_barrier()
+{
+    __threadfence();
+    __syncthreads();
+    __threadfence();
+}
+
+// column pass over an L1 x L2 matrix (row-major, L2 columns), all columns:
+// MODE 0: src -> FFT_L1 -> * tw2 (c*p, DIR) -> dst
+// MODE 1: x[n] * chirp (n < nv, conj if cj) -> FFT -> * tw2 -> dst (Bluestein in)
+// MODE 2: src -> IFFT -> y[n] = conj?(v * chirp[n]) * sc, n < nv (Bluestein out)
+template <typename T, int L1, int L2, int DIR, int MODE>
+__device__ void rg_cols(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, int nv, int cj, T sc, Cx<T> *lds)
+{
+    using S = Sched<L1>;
+    using C = Cx<T>;
+    constexpr int LPB = RG_THREADS / S::NT;
+    static_assert(L2 % LPB == 0, "column tiles must cover the columns");
+    const int b = threadIdx.x % LPB, j = threadIdx.x / LPB;
+    for (int c0 = 0; c0 < L2; c0 += LPB) {
+        const int c = c0 + b;
+        C reg[S::R];
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const int p = j + S::NT * q;
+            const int n = p * L2 + c;
+            if constexpr (MODE == 1) {
+                C v = czero<T>();
+                if (n < nv) {
+                    v = src[n];
+                    if (cj) v = cconj(v);
+                    v = cmul(v, ((const C *)tb.chirp)[n]);
+                }
+                reg[q] = v;
+            } else {
+                reg[q] = src[n];
+            }
+        }
+        LayCol<LPB> lay{b};
+        line_fft<T, L1, DIR>(reg, lds, lay, j, (const double2 *)tb.tw_a);
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const int p = j + S::NT * q;
+            const int n = p * L2 + c;
+            if constexpr (MODE == 2) {
+                if (n < nv) {
+                    C v = cmul(reg[q], ((const C *)tb.chirp)[n]);
+                    if (cj) v = cconj(v);
+                    dst[n] = cscale(v, sc);
+                }
+            }
+        }
+        if constexpr (MODE != 2) {
+            apply_tw2_series<DIR>(reg, tb.tw2, (uint32_t)c * (uint32_t)j, (uint32_t)c * (uint32_t)S::NT);
+#pragma unroll
+            for (int q = 0; q < S::R; q++) dst[(j + S::NT * q) * L2 + c] = reg[q];
+        }
+    }
+}
+
+// row pass: rows of L2 points, L1 rows.
+// MODE 0: FFT_L2 (DIR), store transposed to dst[r + L1*k] (four-step pass 2)
+// MODE 1: FFT, * H[r][k], IFFT, * conj(tw2(r*n2)), store in place (Bluestein middle)
+template <typename T, int L1, int L2, int DIR, int MODE>
+__device__ void rg_rows(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, Cx<T> *lds)
+{
+    using S = Sched<L2>;
+    using C = Cx<T>;
+    constexpr int LPB = RG_THREADS / S::NT;
+    constexpr int PADL = lpadlen<T, L2>();
+    const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
+    for (int r0 = 0; r0 < L1; r0 += LPB) {
+        const int r = r0 + b;
+        C reg[S::R];
+        if constexpr (S::NT >= 64) {
+#pragma unroll
+            for (int q = 0; q < S::R; q++) reg[q] = src[(long long)r * L2 + j + S::NT * q];
+        } else {
+            for (int idx = threadIdx.x; idx < LPB * L2; idx += RG_THREADS) {
+                const int bb = idx / L2, p = idx % L2;
+                lds[bb * PADL + lpad<T>(p)] = src[(long long)(r0 + bb) * L2 + p];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < S::R; q++) reg[q] = lds[b * PADL + lpad<T>(j + S::NT * q)];
+            __syncthreads();
+        }
+        LayLine<T, L2> lay{b * PADL};
+        if constexpr (MODE == 0) {
+            line_fft<T, L2, DIR>(reg, lds, lay, j, (const double2 *)tb.tw_b);
+#pragma unroll
+            for (int q = 0; q < S::R; q++) lds[(j + S::NT * q) * (LPB + 1) + b] = reg[q];
+            __syncthreads();
+            for (int idx = threadIdx.x; idx < LPB * L2; idx += RG_THREADS) {
+                const int bb = idx % LPB, k = idx / LPB;
+                dst[(long long)k * L1 + r0 + bb] = lds[k * (LPB + 1) + bb];
+            }
+            __syncthreads();
+        } else {
+            line_fft<T, L2, -1>(reg, lds, lay, j, (const double2 *)tb.tw_b);
+            const C *Hp = (const C *)tb.H + (long long)r * L2;
+#pragma unroll
+            for (int q = 0; q < S::R; q++) reg[q] = cmul(reg[q], Hp[j + S::NT * q]);
+            line_fft<T, L2, 1>(reg, lds, lay, j, (const double2 *)tb.tw_b);
+            apply_tw2_series<1>(reg, tb.tw2, (uint32_t)r * (uint32_t)j, (uint32_t)r * (uint32_t)S::NT);
+            if constexpr (S::NT >= 64) {
+#pragma unroll
+                for (int q = 0; q < S::R; q++) dst[(long long)r * L2 + j + S::NT * q] = reg[q];
+            } else {
+#pragma unroll
+                for (int q = 0; q < S::R; q++) lds[b * PADL + lpad<T>(j + S::NT * q)] = reg[q];
+                __syncthreads();
+                for (int idx = threadIdx.x; idx < LPB * L2; idx += RG_THREADS) {
+                    const int bb = idx / L2, p = idx % L2;
+                    dst[(long long)(r0 + bb) * L2 + p] = lds[bb * PADL + lpad<T>(p)];
+                }
+                __syncthreads();
+            }
+        }
+    }
+}
+
+template <typename T, int N1, int N2, int DIR>
+__device__ void rg_four(const RagArgs &a, const RagItem &it, Cx<T> *lds)
+{
+    using C = Cx<T>;
+    const RagTab tb = a.tabs[it.tab];
+    const long long base = a.offs[a.ids[it.first]];
+    C *scr = (C *)a.scratch + (long long)blockIdx.x * a.scratch_per_block;
+    rg_cols<T, N1, N2, DIR, 0>((const C *)a.in + base, scr, tb, 0, 0, (T)1, lds);
+    rg_fence_barrier();
+    rg_rows<T, N1, N2, DIR, 0>(scr, (C *)a.out + base, tb, lds);
+    rg_fence_barrier();
+}
+
+template <typename T, int M1, int M2, int DIR>
+__device__ void rg_blue3(const RagArgs &a, const RagItem &it, Cx<T> *lds)
+{
+    using C = Cx<T>;
+    const RagTab tb = a.tabs[it.tab];
+    const long long base = a.offs[a.ids[it.first]];
+    C *scr = (C *)a.scratch + (long long)blockIdx.x * a.scratch_per_block;
+    const int cj = DIR > 0 ? 1 : 0;
+    const T sc = DIR > 0 ? (T)1 / (T)it.n : (T)1;
+    rg_cols<T, M1, M2, -1, 1>((const C *)a.in + base, scr, tb, it.n, cj, (T)1, lds);
+    rg_fence_barrier();
+    rg_rows<T, M1, M2, -1, 1>(scr, scr, tb, lds);
+    rg_fence_barrier();
+    rg_cols<T, M1, M2, 1, 2>(scr, (C *)a.out + base, tb, it.n, cj, sc, lds);
+    rg_fence_barrier();
+}
+
+// ---------------------------------------------------------------------------
+// persistent kernel
+// ---------------------------------------------------------------------------
+template <typename T, int DIR>
+__global__ void __launch_bounds__(RG_THREADS) k_ragged(RagArgs a)
+{
+    extern __shared__ char smem[];
+    Cx<T> *lds = (Cx<T> *)smem;
+    __shared__ long long bases[RG_THREADS];
+    __shared__ int cur;
+    for (;;) {
+        if (threadIdx.x == 0) cur = atomicAdd(a.counter, 1);
+        __syncthreads();
+        const int i = cur;
+        __syncthreads();
+        if (i >= a.nitems) break;  // every workgroup exits once the queue is drained
+        const RagItem it = a.items[i];
+        switch (it.type) {
+        case RG_LINE:
+            switch (it.L) {
+#define NFFT_RG(LL) case LL: rg_line<T, LL, DIR, false>(a, it, lds, bases); break;
+                NFFT_RG(1) NFFT_RG(2) NFFT_RG(4) NFFT_RG(8) NFFT_RG(16) NFFT_RG(32) NFFT_RG(64) NFFT_RG(128)
+                NFFT_RG(256) NFFT_RG(512) NFFT_RG(1024) NFFT_RG(2048) NFFT_RG(4096)
+#undef NFFT_RG
+            }
+            break;
+        case RG_BLUE1:
+            switch (it.L) {
+#define NFFT_RG(LL) case LL: rg_line<T, LL, DIR, true>(a, it, lds, bases); break;
+                NFFT_RG(4) NFFT_RG(8) NFFT_RG(16) NFFT_RG(32) NFFT_RG(64) NFFT_RG(128) NFFT_RG(256) NFFT_RG(512)
+                NFFT_RG(1024) NFFT_RG(2048) NFFT_RG(4096)
+#undef NFFT_RG
+            }
+            break;
+        case RG_FOUR:
+            switch (it.L * 100000 + it.L2) {
+            case 64 * 100000 + 128: rg_four<T, 64, 128, DIR>(a, it, lds); break;
+            case 128 * 100000 + 128: rg_four<T, 128, 128, DIR>(a, it, lds); break;
+            case 128 * 100000 + 256: rg_four<T, 128, 256, DIR>(a, it, lds); break;
+            case 256 * 100000 + 256: rg_four<T, 256, 256, DIR>(a, it, lds); break;
+            }
+            break;
+        case RG_BLUE3:
+            switch (it.L * 100000 + it.L2) {
+            case 64 * 100000 + 128: rg_blue3<T, 64, 128, DIR>(a, it, lds); break;
+            case 128 * 100000 + 128: rg_blue3<T, 128, 128, DIR>(a, it, lds); break;
+            case 128 * 100000 + 256: rg_blue3<T, 128, 256, DIR>(a, it, lds); break;
+            case 256 * 100000 + 256: rg_blue3<T, 256, 256, DIR>(a, it, lds); break;
+            case 512 * 100000 + 256: rg_blue3<T, 512, 256, DIR>(a, it, lds); break;
+            }
+            break;
+        }
+    }
+}
+
+__global__ void k_reset_counter(int *c)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
+}
+
+// LDS bytes of the largest path: 256 threads x 16 points (+ padding / the
+// transposed tile's extra column)
+static size_t rg_lds_bytes(int prec)
+{
+    const size_t e = prec == F32 ? 8 : 16;
+    const size_t pts = (size_t)RG_THREADS * 16;
+    const size_t pad = prec == F32 ? pts / 16 : pts / 8;
+    return (pts + std::max(pad, (size_t)256 + 16)) * e;
+}
+
+}  // namespace nfft
+
+// ===========================================================================
+using namespace nfft;
+
+namespace {
+bool is_pow2(long long n) { return n > 0 && (n & (n - 1)) == 0; }
+int ilog2i(long long n)
+{
+    int l = 0;
+    while ((1LL << l) < n) l++;
+    return l;
+}
+// splits used inside one workgroup (see the kernel's dispatch table)
+bool four_split(int n, int *n1, int *n2)
+{
+    switch (n) {
+    case 8192: *n1 = 64; *n2 = 128; return true;
+    case 16384: *n1 = 128; *n2 = 128; return true;
+    case 32768: *n1 = 128; *n2 = 256; return true;
+    case 65536: *n1 = 256; *n2 = 256; return true;
+    case 131072: *n1 = 512; *n2 = 256; return true;
+    }
+    return false;
+}
+
+struct GraphKey {
+    const void *in;
+    void *out;
+    int inv;
+    bool operator<(const GraphKey &o) const
+    {
+        if (in != o.in) return in < o.in;
+        if (out != o.out) return out < o.out;
+        return inv < o.inv;
+    }
+};
+}  // namespace
+
+struct nfft_ragged {
+    int prec = 0;
+    int count = 0;
+    long long total = 0;
+    std::vector<C2C> engines;  // per distinct size (tables)
+    void *d_items = nullptr, *d_ids = nullptr, *d_offs = nullptr, *d_tabs = nullptr, *d_counter = nullptr;
+    void *scratch = nullptr;
+    long long scratch_per_block = 0;
+    int nitems = 0;
+    int grid = 0;
+    size_t lds = 0;
+    hipStream_t cap = nullptr;
+    std::map<GraphKey, std::pair<hipGraph_t, hipGraphExec_t>> graphs;
+    std::vector<void *> owned;  // per-size Bluestein H buffers
+};
+
+static int launch_ragged(nfft_ragged *r, int inverse, const void *in, void *out, hipStream_t s)
+{
+    RagArgs a{in, out, (const RagItem *)r->d_items, r->nitems, (const int *)r->d_ids,
+              (const long long *)r->d_offs, (const RagTab *)r->d_tabs, (int *)r->d_counter, r->scratch,
+              r->scratch_per_block};
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, (int *)r->d_counter);
+    if (r->prec == F32) {
+        if (inverse) hipLaunchKernelGGL((k_ragged<float, 1>), dim3(r->grid), dim3(RG_THREADS), r->lds, s, a);
+        else hipLaunchKernelGGL((k_ragged<float, -1>), dim3(r->grid), dim3(RG_THREADS), r->lds, s, a);
+    } else {
+        if (inverse) hipLaunchKernelGGL((k_ragged<double, 1>), dim3(r->grid), dim3(RG_THREADS), r->lds, s, a);
+        else hipLaunchKernelGGL((k_ragged<double, -1>), dim3(r->grid), dim3(RG_THREADS), r->lds, s, a);
+    }
+    return hipGetLastError() == hipSuccess ? NFFT_OK : NFFT_E_HIP;
+}
+
+static void ragged_free(nfft_ragged *r)
+{
+    for (auto &kv : r->graphs) {
+        (void)hipGraphExecDestroy(kv.second.second);
+        (void)hipGraphDestroy(kv.second.first);
+    }
+    for (auto &e : r->engines) e.release();
+    for (void *p : r->owned) (void)hipFree(p);
+    void *ptrs[] = {r->d_items, r->d_ids, r->d_offs, r->d_tabs, r->d_counter, r->scratch};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (r->cap) (void)hipStreamDestroy(r->cap);
+    delete r;
+}
+
+// Workgroups per CU for the persistent grid: from the tuning cache file
+// (NFFT_TUNE_CACHE) when present, else 4 (FP32) / 2 (FP64).
+static std::string tune_key(int prec, const std::vector<int> &sizes)
+{
+    unsigned long long h = 1469598103934665603ULL;
+    for (int v : sizes) h = (h ^ (unsigned long long)v) * 1099511628211ULL;
+    char b[96];
+    snprintf(b, sizeof b, "ragged p%d c%zu h%016llx", prec, sizes.size(), h);
+    return b;
+}
+
+static int tune_lookup(const std::string &key)
+{
+    const char *path = getenv("NFFT_TUNE_CACHE");
+    if (!path || !*path) return 0;
+    FILE *f = fopen(path, "r");
+    if (!f) return 0;
+    char line[256];
+    int found = 0;
+    const size_t kl = key.size();
+    while (fgets(line, sizeof line, f)) {
+        if (strncmp(line, key.c_str(), kl) == 0 && line[kl] == ' ') found = atoi(line + kl + 1);
+    }
+    fclose(f);
+    return found;
+}
+
+static void tune_store(const std::string &key, int val)
+{
+    const char *path = getenv("NFFT_TUNE_CACHE");
+    if (!path || !*path) return;
+    FILE *f = fopen(path, "a");
+    if (!f) return;
+    fprintf(f, "%s %d\n", key.c_str(), val);
+    fclose(f);
+}
+
+extern "C" {
+
+int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes, int flags)
+{
+    clear_error();
+    if (!out || count < 1 || !sizes || (prec != NFFT_F32 && prec != NFFT_F64)) {
+        set_error("invalid ragged arguments");
+        return NFFT_E_INVALID;
+    }
+    *out = nullptr;
+    nfft_ragged *r = new (std::nothrow) nfft_ragged();
+    if (!r) return NFFT_E_ALLOC;
+    r->prec = prec;
+    r->count = count;
+    const size_t cb = cx_bytes(prec);
+    std::vector<long long> offs(count);
+    std::map<int, int> size_tab;  // size -> table index
+    std::vector<RagTab> tabs;
+    long long max_scratch = 0;
+    for (int i = 0; i < count; i++) {
+        const int n = sizes[i];
+        if (n < 1) {
+            set_error("ragged sizes must be >= 1");
+            ragged_free(r);
+            return NFFT_E_SIZE;
+        }
+        offs[i] = r->total;
+        r->total += n;
+        if (size_tab.count(n)) continue;
+        // tables for this size (twiddles, chirp, H) come from a plain engine
+        C2C e;
+        const bool p2 = is_pow2(n);
+        long long m = 1;
+        while (m < 2LL * n - 1) m <<= 1;
+        if ((p2 && n > 65536) || (!p2 && m > 131072)) {
+            set_error("ragged batches support N <= 65536");
+            ragged_free(r);
+            return NFFT_E_UNSUPPORTED;
+        }
+        RagTab t{};
+        if (p2 && n <= 4096) {
+            t.tw_a = stage_table(prec, n);
+            if (!t.tw_a) { ragged_free(r); return NFFT_E_HIP; }
+        } else if (p2) {
+            int n1, n2;
+            four_split(n, &n1, &n2);
+            t.tw_a = stage_table(prec, n1);
+            t.tw_b = stage_table(prec, n2);
+            if (!t.tw_a || !t.tw_b || !tw2_table(n, &t.tw2)) { ragged_free(r); return NFFT_E_HIP; }
+            max_scratch = std::max(max_scratch, (long long)n);
+        } else {
+            // Bluestein: the engine builds chirp and H in the layout it uses
+            // for M <= line limit (natural) or M1 x M2 (permuted); the ragged
+            // kernel needs natural order for M <= 4096 and the [k1][k2] order
+            // for its own M1 x M2 split, so H is rebuilt here when they differ.
+            if (!e.init(n, prec)) { ragged_free(r); return NFFT_E_HIP; }
+            t.chirp = e.chirp;
+            if (m <= 4096) {
+                t.tw_a = stage_table(prec, (int)m);
+                if (e.algo != ALG_BLUE1) { set_error("unexpected Bluestein layout"); ragged_free(r); return NFFT_E_HIP; }
+                t.H = e.H;
+            } else {
+                int m1, m2;
+                four_split((int)m, &m1, &m2);
+                t.tw_a = stage_table(prec, m1);
+                t.tw_b = stage_table(prec, m2);
+                if (!t.tw_a || !t.tw_b || !tw2_table(m, &t.tw2)) { ragged_free(r); return NFFT_E_HIP; }
+                // permute H (natural or engine-permuted) into [k1][k2] for m1 x m2
+                void *Hn = nullptr;
+                if (hipMalloc(&Hn, (size_t)m * cb) != hipSuccess) { ragged_free(r); return NFFT_E_ALLOC; }
+                r->owned.push_back(Hn);
+                // recompute H in FP64 from the definition with this split
+                C2C h64;
+                void *hb = nullptr, *Hb = nullptr, *sb = nullptr;
+                const double2 *chirp64 = nullptr;
+                chirp_table(prec, n, &chirp64);
+                hipStream_t st = nullptr;
+                (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+                bool ok = h64.init((int)m, F64) && hipMalloc(&hb, (size_t)m * 16) == hipSuccess &&
+                          hipMalloc(&Hb, (size_t)m * 16) == hipSuccess;
+                if (ok && h64.scratch_per_line) ok = hipMalloc(&sb, (size_t)h64.scratch_per_line * 16) == hipSuccess;
+                ok = ok && launch_blue_filter((double2 *)hb, chirp64, n, (int)m, st) == hipSuccess &&
+                     h64.run(hb, Hb, 1, m, m, -1, 1.0, sb, 1, st) == hipSuccess &&
+                     launch_blue_hprep((const double2 *)Hb, Hn, prec, (int)m, m1, m2, 1.0 / (double)m, st) ==
+                         hipSuccess &&
+                     hipStreamSynchronize(st) == hipSuccess;
+                if (hb) (void)hipFree(hb);
+                if (Hb) (void)hipFree(Hb);
+                if (sb) (void)hipFree(sb);
+                h64.release();
+                if (st) (void)hipStreamDestroy(st);
+                if (!ok) { set_error("ragged Bluestein table setup failed"); ragged_free(r); return NFFT_E_HIP; }
+                t.H = Hn;
+                max_scratch = std::max(max_scratch, m);
+            }
+            r->engines.push_back(e);  // keeps chirp/H alive
+        }
+        size_tab[n] = (int)tabs.size();
+        tabs.push_back(t);
+    }
+    // work items
+    std::map<std::pair<int, int>, std::vector<int>> groups;  // (type, size) -> ids
+    for (int i = 0; i < count; i++) {
+        const int n = sizes[i];
+        int type;
+        if (is_pow2(n)) type = n <= 4096 ? RG_LINE : RG_FOUR;
+        else {
+            long long m = 1;
+            while (m < 2LL * n - 1) m <<= 1;
+            type = m <= 4096 ? RG_BLUE1 : RG_BLUE3;
+        }
+        groups[{type, n}].push_back(i);
+    }
+    std::vector<RagItem> items;
+    std::vector<int> ids;
+    std::vector<double> cost;
+    for (auto &kv : groups) {
+        const int type = kv.first.first, n = kv.first.second;
+        const std::vector<int> &v = kv.second;
+        long long m = 1;
+        while (m < 2LL * n - 1) m <<= 1;
+        int L = n, L2 = 0, per = 1;
+        double c1 = 0;
+        if (type == RG_LINE) { L = n; per = RG_THREADS / std::max(1, L / 16); c1 = 2.0 * n; }
+        else if (type == RG_BLUE1) { L = (int)m; per = RG_THREADS / std::max(1, L / 16); c1 = 2.0 * n + 0.5 * m; }
+        else if (type == RG_FOUR) { four_split(n, &L, &L2); c1 = 4.0 * n; }
+        else { four_split((int)m, &L, &L2); c1 = 6.0 * m; }
+        if (L < 16 && (type == RG_LINE || type == RG_BLUE1)) per = RG_THREADS;
+        for (size_t k = 0; k < v.size(); k += per) {
+            RagItem it{};
+            it.type = type;
+            it.L = L;
+            it.L2 = L2;
+            it.n = n;
+            it.count = (int)std::min<size_t>(per, v.size() - k);
+            it.first = (int)ids.size();
+            it.tab = size_tab[n];
+            for (size_t q = k; q < k + it.count; q++) ids.push_back(v[q]);
+            items.push_back(it);
+            cost.push_back(c1 * it.count);
+        }
+    }
+    // longest first
+    std::vector<int> order(items.size());
+    for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<RagItem> sorted;
+    for (int o : order) sorted.push_back(items[o]);
+    r->nitems = (int)sorted.size();
+    int dev = 0, ncu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    r->lds = rg_lds_bytes(prec);
+    {
+        const int lds = (int)r->lds;
+        hipError_t e1, e2;
+        if (prec == F32) {
+            e1 = hipFuncSetAttribute((const void *)k_ragged<float, -1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+            e2 = hipFuncSetAttribute((const void *)k_ragged<float, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        } else {
+            e1 = hipFuncSetAttribute((const void *)k_ragged<double, -1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+            e2 = hipFuncSetAttribute((const void *)k_ragged<double, 1>, hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        }
+        (void)e1;
+        (void)e2;
+        (void)hipGetLastError();
+    }
+    const std::string key = tune_key(prec, std::vector<int>(sizes, sizes + count));
+    int per_cu = tune_lookup(key);
+    if (per_cu <= 0) per_cu = prec == F32 ? 4 : 2;
+    r->grid = std::max(1, std::min(r->nitems, ncu * per_cu));
+    r->scratch_per_block = max_scratch;
+    const int max_grid = ncu * 8;
+    bool ok = hipMalloc(&r->d_items, sorted.size() * sizeof(RagItem)) == hipSuccess &&
+              hipMalloc(&r->d_ids, ids.size() * sizeof(int)) == hipSuccess &&
+              hipMalloc(&r->d_offs, offs.size() * sizeof(long long)) == hipSuccess &&
+              hipMalloc(&r->d_tabs, tabs.size() * sizeof(RagTab)) == hipSuccess &&
+              hipMalloc(&r->d_counter, 64) == hipSuccess;
+    if (ok && max_scratch)
+        ok = hipMalloc(&r->scratch, (size_t)max_grid * max_scratch * cb) == hipSuccess;
+    ok = ok && hipMemcpy(r->d_items, sorted.data(), sorted.size() * sizeof(RagItem), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(r->d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(r->d_offs, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice) == hipSuccess &&
+         hipMemcpy(r->d_tabs, tabs.data(), tabs.size() * sizeof(RagTab), hipMemcpyHostToDevice) == hipSuccess &&
+         hipStreamCreateWithFlags(&r->cap, hipStreamNonBlocking) == hipSuccess;
+    if (!ok) {
+        set_error("ragged setup: device allocation or copy failed");
+        ragged_free(r);
+        return NFFT_E_ALLOC;
+    }
+    if (flags & 1) {
+        // project-written autotuner: time the persistent launch for a few
+        // workgroups-per-CU choices on a scratch batch and keep the fastest
+        void *tin = nullptr, *tout = nullptr;
+        if (hipMalloc(&tin, (size_t)r->total * cb) == hipSuccess && hipMalloc(&tout, (size_t)r->total * cb) == hipSuccess) {
+            (void)launch_fill_uniform(tin, 2 * r->total, 7, 0, prec, r->cap);
+            hipEvent_t e0, e1;
+            (void)hipEventCreate(&e0);
+            (void)hipEventCreate(&e1);
+            float best = 1e30f;
+            int best_pc = per_cu;
+            const int cands[] = {1, 2, 3, 4, 6, 8};
+            for (int pc : cands) {
+                if (prec == F64 && pc > 4) continue;
+                r->grid = std::max(1, std::min(r->nitems, ncu * pc));
+                (void)launch_ragged(r, 0, tin, tout, r->cap);
+                (void)hipEventRecord(e0, r->cap);
+                for (int k = 0; k < 3; k++) (void)launch_ragged(r, 0, tin, tout, r->cap);
+                (void)hipEventRecord(e1, r->cap);
+                (void)hipEventSynchronize(e1);
+                float ms = 0;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms < best) { best = ms; best_pc = pc; }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            per_cu = best_pc;
+            r->grid = std::max(1, std::min(r->nitems, ncu * per_cu));
+            tune_store(key, per_cu);
+        }
+        if (tin) (void)hipFree(tin);
+        if (tout) (void)hipFree(tout);
+    }
+    *out = r;
+    return NFFT_OK;
+}
+
+int nfft_ragged_execute(nfft_ragged *r, int inverse, const void *in, void *out, void *stream)
+{
+    clear_error();
+    if (!r || !in || !out) {
+        set_error("NULL ragged handle or buffer");
+        return NFFT_E_INVALID;
+    }
+    GraphKey k{in, out, inverse ? 1 : 0};
+    auto it = r->graphs.find(k);
+    if (it == r->graphs.end()) {
+        if (r->graphs.size() >= 8) {
+            (void)hipGraphExecDestroy(r->graphs.begin()->second.second);
+            (void)hipGraphDestroy(r->graphs.begin()->second.first);
+            r->graphs.erase(r->graphs.begin());
+        }
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        if (hipStreamBeginCapture(r->cap, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            set_error("graph capture failed to start");
+            return NFFT_E_HIP;
+        }
+        const int st = launch_ragged(r, k.inv, in, out, r->cap);
+        if (hipStreamEndCapture(r->cap, &g) != hipSuccess || st != NFFT_OK ||
+            hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            set_error("graph capture of the ragged launch failed");
+            return NFFT_E_HIP;
+        }
+        it = r->graphs.emplace(k, std::make_pair(g, ge)).first;
+    }
+    if (hipGraphLaunch(it->second.second, (hipStream_t)stream) != hipSuccess) {
+        set_error("graph launch failed");
+        return NFFT_E_HIP;
+    }
+    return NFFT_OK;
+}
+
+int nfft_ragged_destroy(nfft_ragged *r)
+{
+    if (r) ragged_free(r);
+    return NFFT_OK;
+}
+
+long long nfft_ragged_total(const nfft_ragged *r) { return r ? r->total : 0; }
+
+}  // extern "C"

@@ -35,6 +35,7 @@ def run_plan(pkg, dev, op, dims, x, batch=1, prec="f32", norm="reference", vjp=F
     else:
         plan.execute(op, xin.data_ptr(), out.data_ptr())
     torch.cuda.synchronize()
+    plan.check()  # no device-side dependency wait timed out
     plan.close()
     return out.cpu().numpy().astype(np.float64)
 

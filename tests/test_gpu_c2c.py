@@ -174,3 +174,65 @@ def test_dropin_f32(pkg, dev):
         lib.notorious_fft_free_aux(a)
         ref = ol.run_oracle("dft", [n], x.astype(np.float64))
         assert ol.rel_l2(y.astype(np.float64), ref) < tol_for("f32", n)
+
+
+@pytest.mark.parametrize("prec,n,batch", [("f32", 1 << 20, 5), ("f64", 1 << 18, 7), ("f32", 1 << 15, 9)])
+@pytest.mark.parametrize("persist", ["1", "0"])
+def test_four_step_small_scratch(pkg, dev, prec, n, batch, persist, monkeypatch):
+    """A 1 MiB scratch: the persistent four-step then runs with a single
+    rotating slot (every pass-1 tile waits for the previous transform's pass
+    2), the two-launch path with many chunks; in place too."""
+    monkeypatch.setenv("NFFT_CHUNK_MB", "1")
+    monkeypatch.setenv("NFFT_PERSIST", persist)
+    x = rounded(ol.uniform(7, 2 * n * batch), prec)
+    ref = ol.run_oracle("dft", [n], x, batch)
+    for inplace in (False, True):
+        got = run_plan(pkg, dev, "dft", [n], x, batch, prec, inplace=inplace)
+        assert ol.rel_l2(got, ref) < tol_for(prec, n), (inplace, ol.rel_l2(got, ref))
+    refi = ol.run_oracle("invdft", [n], x, batch)
+    got = run_plan(pkg, dev, "invdft", [n], x, batch, prec)
+    assert ol.rel_l2(got, refi) < tol_for(prec, n)
+
+
+def persist_variants(prec, n):
+    """Shapes built for length n (variant 0 is the default)."""
+    return {("f32", 1 << 20): [0, 1, 2, 3, 4, 5], ("f64", 1 << 20): [0, 1]}.get((prec, n), [0])
+
+
+@pytest.mark.parametrize("prec,n,batch", [("f32", 1 << 20, 3), ("f64", 1 << 20, 2)])
+def test_persist_variants(pkg, dev, prec, n, batch, monkeypatch):
+    """Every alternative tile shape of the persistent four-step."""
+    monkeypatch.setenv("NFFT_PERSIST", "1")
+    x = rounded(ol.uniform(8, 2 * n * batch), prec)
+    ref = ol.run_oracle("dft", [n], x, batch)
+    for v in persist_variants(prec, n):
+        monkeypatch.setenv("NFFT_PERSIST_VAR", str(v))
+        got = run_plan(pkg, dev, "dft", [n], x, batch, prec)
+        err = ol.rel_l2(got, ref)
+        print(f"{prec} N={n} variant {v}: rel-L2 {err:.3e}")
+        assert err < tol_for(prec, n), (v, err)
+
+
+def test_persist_many_transforms(pkg, dev, monkeypatch):
+    """More transforms than rotating slots and a batch that is not a multiple
+    of anything: N = 2^15 x 300 with a 2 MiB scratch (8 slots)."""
+    n, batch = 1 << 15, 300
+    x = rounded(ol.uniform(9, 2 * n * batch), "f32")
+    ref = ol.run_oracle("dft", [n], x, batch)
+    monkeypatch.setenv("NFFT_CHUNK_MB", "2")
+    monkeypatch.setenv("NFFT_PERSIST", "1")
+    got = run_plan(pkg, dev, "dft", [n], x, batch, "f32")
+    assert ol.rel_l2(got, ref) < tol_for("f32", n)
+
+
+@pytest.mark.parametrize("grab", [2, 5])
+def test_persist_grab(pkg, dev, grab, monkeypatch):
+    """Several queue items per access (runs that straddle the two passes and
+    the end of the queue)."""
+    monkeypatch.setenv("NFFT_PERSIST", "1")
+    monkeypatch.setenv("NFFT_PERSIST_GRAB", str(grab))
+    for prec, n, batch in (("f32", 1 << 16, 7), ("f64", 1 << 17, 3)):
+        x = rounded(ol.uniform(10, 2 * n * batch), prec)
+        for op in ("dft", "invdft"):
+            got = run_plan(pkg, dev, op, [n], x, batch, prec)
+            assert ol.rel_l2(got, ol.run_oracle(op, [n], x, batch)) < tol_for(prec, n), (prec, op)

@@ -1,0 +1,100 @@
+"""Ragged batches (one persistent launch replayed from a HIP graph): every
+transform against pocketfft (numpy.fft, FP64), the replays against eager
+per-size plans, with the program's own work between replays."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+from gpu_util import rounded, tol_for
+from ragged_sizes import ragged_sizes
+
+pytestmark = pytest.mark.gpu
+
+
+def make(pkg, prec, sizes, flags=0):
+    lib = pkg.core()
+    h = ctypes.c_void_p()
+    arr = (ctypes.c_int * len(sizes))(*sizes)
+    pkg.check(lib.nfft_ragged_create(ctypes.byref(h), 0 if prec == "f32" else 1, len(sizes), arr, flags),
+              "ragged_create")
+    return h
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_ragged_vs_pocketfft(pkg, dev, prec):
+    import torch
+    sizes = ragged_sizes(300, seed=1)
+    sizes += [1, 2, 3, 16, 4096, 4097, 8192, 65536, 65521, 2 * 3 * 5 * 7]
+    lib = pkg.core()
+    h = make(pkg, prec, sizes)
+    total = lib.nfft_ragged_total(h)
+    assert total == sum(sizes)
+    x = rounded(ol.uniform(51, 2 * total), prec)
+    dt = torch.float32 if prec == "f32" else torch.float64
+    xd = torch.from_numpy(x.astype(np.float32 if prec == "f32" else np.float64)).to(dev)
+    yd = torch.empty_like(xd)
+    for inverse in (0, 1):
+        torch.cuda.synchronize()
+        pkg.check(lib.nfft_ragged_execute(h, inverse, xd.data_ptr(), yd.data_ptr(), None), "ragged_execute")
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy().astype(np.float64)
+        off = 0
+        worst = 0.0
+        for n in sizes:
+            z = x[2 * off:2 * (off + n)]
+            zc = z[0::2] + 1j * z[1::2]
+            if inverse:
+                want = np.fft.ifft(zc) * (n if (n & (n - 1)) == 0 else 1)
+            else:
+                want = np.fft.fft(zc)
+            got = y[2 * off:2 * (off + n)]
+            err = ol.rel_l2(got[0::2] + 1j * got[1::2], want) if n > 1 else abs(got[0] - want.real[0])
+            worst = max(worst, err / tol_for(prec, n))
+            assert err < tol_for(prec, n), (prec, inverse, n, err)
+            off += n
+        print(f"{prec} inverse={inverse}: worst error / tolerance = {worst:.3f}")
+    assert lib.nfft_ragged_destroy(h) == 0
+    del dt
+
+
+def test_ragged_graph_replays_match_eager(pkg, dev):
+    import torch
+    sizes = ragged_sizes(200, seed=2)
+    lib = pkg.core()
+    h = make(pkg, "f32", sizes)
+    total = sum(sizes)
+    xd = torch.empty(2 * total, dtype=torch.float32, device=dev)
+    pkg.fill_uniform(xd.data_ptr(), xd.numel(), 3, 0, "f32")
+    # eager reference: one batched plan per distinct size, transform by transform
+    ref = torch.empty_like(xd)
+    off = 0
+    plans = {}
+    for n in sizes:
+        if n not in plans:
+            plans[n] = pkg.Plan("dft", "f32", [n], 1)
+        plans[n].execute("dft", xd[2 * off:].data_ptr(), ref[2 * off:].data_ptr())
+        off += n
+    torch.cuda.synchronize()
+    for rep in range(4):
+        yd = torch.full_like(xd, float("nan"))
+        junk = torch.randn(1 << 20, device=dev).sum()  # program's own work between replays
+        torch.cuda.synchronize()
+        pkg.check(lib.nfft_ragged_execute(h, 0, xd.data_ptr(), yd.data_ptr(), None), "ragged_execute")
+        torch.cuda.synchronize()
+        d = (yd - ref).abs().max().item()
+        assert d <= 1e-4 * ref.abs().max().item(), (rep, d)
+        del junk
+    for p in plans.values():
+        p.close()
+    lib.nfft_ragged_destroy(h)
+
+
+def test_ragged_rejects_oversize(pkg):
+    lib = pkg.core()
+    h = ctypes.c_void_p()
+    arr = (ctypes.c_int * 2)(16, 131072)
+    assert lib.nfft_ragged_create(ctypes.byref(h), 0, 2, arr, 0) == pkg.E_UNSUPPORTED
+    arr = (ctypes.c_int * 1)(0)
+    assert lib.nfft_ragged_create(ctypes.byref(h), 0, 1, arr, 0) == pkg.E_SIZE
