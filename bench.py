@@ -15,9 +15,10 @@ Also printed in the same JSON line:
                 on a bounded sample of the same workload on this host
   sweep         the rest of the batched C2C configuration (N = 2^10..2^20,
                 FP32 and FP64, 2^28 elements each) with rel-L2 vs the oracle
+  sweep_r2c     batched realdft N = 2^10..2^20, FP32 and FP64, 2^28 reals each
   accuracy      rel-L2 of sampled output transforms vs the FP64 oracle
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-sweep] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-sweep] [--no-cpu] [--calib]
 """
 import argparse
 import json
@@ -94,6 +95,34 @@ def run_config(pkg, torch, n, prec, total_elems, steps, warmup, profile=False, c
     del x, y
     torch.cuda.empty_cache()
     return res
+
+
+def run_r2c(pkg, torch, n, prec, total_reals, steps=5, warmup=2):
+    """Batched realdft of length n over total_reals reals (FFTW layout out:
+    n/2+1 complex per transform); rel-L2 of one transform vs the oracle."""
+    import numpy as np
+    import oracle_lib as ol
+    dt = torch.float32 if prec == "f32" else torch.float64
+    eb = 4 if prec == "f32" else 8
+    batch = total_reals // n
+    dev = torch.device("cuda:0")
+    x = torch.empty(n * batch, dtype=dt, device=dev)
+    y = torch.empty((n + 2) * batch, dtype=dt, device=dev)
+    pkg.fill_uniform(x.data_ptr(), x.numel(), 0, 0, prec)
+    plan = pkg.Plan("realdft", prec, [n], batch)
+    t = timed(torch, lambda: plan.execute("realdft", x.data_ptr(), y.data_ptr()), steps, warmup)
+    xs = x[:n].cpu().numpy().astype(np.float64)
+    ys = y[:n + 2].cpu().numpy().astype(np.float64)
+    err = ol.rel_l2(ys, ol.run_oracle("realdft", [n], xs))
+    out = {"n": n, "prec": prec, "batch": batch, "ms": round(1e3 * t, 4),
+           "gflops": round(2.5 * n * math.log2(n) * batch / t / 1e9, 1),
+           "alg_gbs": round((2 * n + 2) * batch * eb / t / 1e9, 1),
+           "hbm_frac": round((2 * n + 2) * batch * eb / t / 1e9 / HBM_PEAK_GBS, 4),
+           "rel_l2": float(f"{err:.3e}"), "plan": plan.describe()}
+    plan.close()
+    del x, y
+    torch.cuda.empty_cache()
+    return out
 
 
 def timed(torch, fn, steps=5, warmup=2):
@@ -181,7 +210,64 @@ def config_extras(pkg, torch):
         lib.nfft_ragged_destroy(h)
         del x, y
         torch.cuda.empty_cache()
+        # the same transforms through the plain path: one batched plan per
+        # distinct size, executed back to back on the stream
+        groups = {}
+        for m in sizes:
+            groups[m] = groups.get(m, 0) + 1
+        plans = [(pkg.Plan("dft", prec, [m], c), m, c) for m, c in sorted(groups.items())]
+        x = torch.empty(2 * total, dtype=dt, device=dev)
+        y = torch.empty_like(x)
+        pkg.fill_uniform(x.data_ptr(), x.numel(), 0, 0, prec)
+
+        def plain():
+            off = 0
+            for p, m, c in plans:
+                p.execute("dft", x.data_ptr() + off * eb, y.data_ptr() + off * eb)
+                off += m * c
+        t = timed(torch, plain, steps=3, warmup=1)
+        out[f"ragged_plain_{prec}_10000"] = {"ms": round(1e3 * t, 4), "gflops": round(flops / t / 1e9, 1),
+                                             "alg_gbs": round(2 * total * eb / t / 1e9, 1),
+                                             "plans": len(plans)}
+        log(f"  ragged plain {prec}: {out[f'ragged_plain_{prec}_10000']}")
+        for p, _, _ in plans:
+            p.close()
+        del x, y
+        torch.cuda.empty_cache()
+    out["ragged_cpu_sample"] = ragged_cpu(sizes)
+    log(f"  ragged cpu: {out['ragged_cpu_sample']}")
     return out
+
+
+def ragged_cpu(sizes, budget=2000):
+    """The reference (FP64) on a bounded random sample of the ragged list:
+    one aux per transform made beforehand, the transforms OpenMP-parallel
+    over the list; scaled to the whole list by nominal flops."""
+    import random
+    import oracle_lib as ol
+    cores = min(16, os.cpu_count() or 1)
+    flags = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
+    name = "libnfft_ref_fast64.so" if ("avx512f" in flags and "avx512dq" in flags) else "libnfft_ref_fast64_avx2.so"
+    rnd = random.Random(0)
+    sample = rnd.sample(sizes, min(budget, len(sizes)))
+    x = ol.uniform(0, 2 * sum(sample))
+    try:
+        _, t = ol.Ref(name).run_list(sample, x, nthreads=cores)
+        kind = "reference"
+    except (FileNotFoundError, OSError):
+        kind, name = "port", "oracle"
+        t0 = time.perf_counter()
+        off = 0
+        for m in sample:
+            ol.run_oracle("dft", [m], x[off:off + 2 * m])
+            off += 2 * m
+        t = time.perf_counter() - t0
+    fl_s = sum(5.0 * m * math.log2(m) for m in sample)
+    fl_all = sum(5.0 * m * math.log2(m) for m in sizes)
+    return {"value_gflops": round(fl_s / t / 1e9, 3), "est_full_ms": round(1e3 * t * fl_all / fl_s, 1),
+            "cores": cores if kind == "reference" else 1, "kind": kind,
+            "sample": f"{len(sample)} of {len(sizes)} transforms (seeded random), {name}, "
+                      f"{t:.2f} s in the transform loop (auxes made beforehand)"}
 
 
 def cpu_baseline(n=1 << 20, budget_s=10.0):
@@ -197,20 +283,20 @@ def cpu_baseline(n=1 << 20, budget_s=10.0):
     kind = "reference"
     try:
         ref = ol.Ref(name)
-        run = lambda x, b: ref.run("dft", [n], x, batch=b, nthreads=cores)  # noqa: E731
+        run = lambda x, b: ref.run_timed("dft", [n], x, batch=b, nthreads=cores)[1]  # noqa: E731
     except (FileNotFoundError, OSError):
         kind, name = "port", "oracle"
-        run = lambda x, b: ol.run_oracle("dft", [n], x, batch=b, nthreads=cores)  # noqa: E731
+
+        def run(x, b):
+            t0 = time.perf_counter()
+            ol.run_oracle("dft", [n], x, batch=b, nthreads=cores)
+            return time.perf_counter() - t0
     x1 = ol.uniform(0, 2 * n * cores)
-    t0 = time.perf_counter()
-    run(x1, cores)
-    per_round = time.perf_counter() - t0
+    per_round = run(x1, cores)
     rounds = max(1, min(8, int(budget_s / max(per_round, 1e-3))))
     batch = rounds * cores
     x = ol.uniform(0, 2 * n * batch)
-    t0 = time.perf_counter()
-    run(x, batch)
-    t = time.perf_counter() - t0
+    t = run(x, batch)
     gflops = 5.0 * n * math.log2(n) * batch / t / 1e9
     cpu_model = ""
     for line in flags.splitlines():
@@ -219,8 +305,9 @@ def cpu_baseline(n=1 << 20, budget_s=10.0):
             break
     del ctypes
     return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": cores, "kind": kind,
-            "sample": f"{batch} x N=2^20 FP64 C2C forward ({name}, OpenMP over the batch, "
-                      f"{t:.2f} s wall); host {cpu_model}; nproc {os.cpu_count()}",
+            "sample": f"{batch} x N=2^20 FP64 C2C forward ({name}, OpenMP over the batch, one aux "
+                      f"per thread, {t:.2f} s in the transform loop); host {cpu_model}; "
+                      f"nproc {os.cpu_count()}",
             "alg_gbs": round(2 * n * 16 * batch / t / 1e9, 3)}
 
 
@@ -232,6 +319,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--calib", action="store_true",
+                    help="first run a plain device copy of 1 GiB (reference kernel for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -242,6 +331,14 @@ def main():
     import __graft_entry__ as ge
 
     pkg = ge.load_package()
+    if args.calib:
+        a = torch.empty(1 << 28, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)  # reads 1 GiB, writes 1 GiB per launch
+        torch.cuda.synchronize()
+        del a, b
+        torch.cuda.empty_cache()
     total = 1 << 28
     log(f"headline: N={args.n} FP32 batch {total // args.n}, warmup {args.warmup}, steps {args.steps}")
     head = run_config(pkg, torch, args.n, "f32", total, args.steps, args.warmup, profile=True, check=2)
@@ -255,6 +352,15 @@ def main():
                 "avg_launch_us": round(dk["avg_us"], 2), "bytes_per_launch": dk["bytes_per_launch"],
                 "kernels": {k: {"avg_us": round(v["avg_us"], 2), "launches": v["launches"],
                                 "gbs": round(v["gbs"], 1)} for k, v in kern.items()}}
+
+    sweep_r2c = []
+    if not args.no_sweep:
+        for prec in ("f32", "f64"):
+            for lg in range(10, 21):
+                r = run_r2c(pkg, torch, 1 << lg, prec, 1 << 28)
+                sweep_r2c.append(r)
+                log(f"  sweep r2c {prec} N=2^{lg}: {r['gflops']:.1f} GFLOP/s {100 * r['hbm_frac']:.1f}% HBM "
+                    f"rel-L2 {r['rel_l2']:.2e} [{r['plan']}]")
 
     sweep = []
     if not args.no_sweep:
@@ -305,6 +411,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "sweep": sweep,
+        "sweep_r2c": sweep_r2c,
         "configs": extras,
     }
     print(json.dumps(out), flush=True)

@@ -12,7 +12,11 @@
 // step s holds the A column tiles of transform s (pass 1) followed by the B
 // row tiles of transform s - D (pass 2).  A pass-2 tile of transform t waits
 // until all A pass-1 tiles of t are published; a pass-1 tile of t waits until
-// the B pass-2 tiles of t - D - 1 (the previous user of its slot) are done.
+// the B pass-2 tiles of t - S (the previous user of its slot, S > D slots)
+// are done.  Both waits then reach back several queue steps (D and S - D - 1),
+// so that in steady state they are already satisfied: with D = S - 1 a pass-1
+// tile would wait on pass-2 tiles handed out one step earlier and the queue
+// would advance at one item latency per step.
 // Every wait points to items handed out earlier, held by workgroups that are
 // already running and that only wait on items earlier still, so the chain
 // always ends at an item that can proceed: the queue cannot deadlock, whether
@@ -44,7 +48,8 @@ struct PersistArgs {
     long long ntrans;
     long long in_dist, out_dist;  // per-transform distances (elements)
     void *scratch;                // slots * N elements
-    int slots;                    // D + 1
+    int slots;                    // rotating intermediate slots S
+    int lag;                      // D < S: pass 2 of transform t runs in queue step t + D
     unsigned *ctr;                // see persist_ctr_words
     const void *tw_a, *tw_b;      // stage tables of N1 and N2
     Tw2 tw2;                      // e^{-2 pi i e / N}
@@ -189,7 +194,7 @@ k_four_persist(PersistArgs a)
                     sub = 0;
                 }
                 const long long i = base + sub++;
-                if (persist_decode(i, ntr, a.slots - 1, Sh::A, Sh::B, &nk, &nt, &ntile)) {
+                if (persist_decode(i, ntr, a.lag, Sh::A, Sh::B, &nk, &nt, &ntile)) {
                     if (nk == 2) persist_wait(done1 + nt, (unsigned)Sh::A, err, a.timeout);
                     else if (nt >= a.slots) persist_wait(done2 + (nt - a.slots), (unsigned)Sh::B, err, a.timeout);
                 }

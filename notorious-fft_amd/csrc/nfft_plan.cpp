@@ -254,6 +254,7 @@ static bool persist_setup(C2C &c)
     c.pgrid = std::max(1, ncu) * c.pcfg.per_cu;
     c.ptimeout = (long long)(kPersistTimeoutSec * 1000.0 * std::max(khz, 1000));
     c.pgrab = std::max(1, env_int("NFFT_PERSIST_GRAB", 1));
+    c.plag = std::max(0, env_int("NFFT_PERSIST_LAG", 0));
     return true;
 }
 
@@ -412,6 +413,7 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
                 a.out_dist = out_dist;
                 a.scratch = scratch;
                 a.slots = (int)std::min(std::min(scratch_lines, cnt), 1LL << 20);
+                a.lag = plag > 0 ? std::min(plag, a.slots - 1) : a.slots / 2;
                 a.ctr = pctr;
                 a.tw_a = tw_a;
                 a.tw_b = tw_b;

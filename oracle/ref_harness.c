@@ -10,6 +10,7 @@
  */
 #define NOTORIOUS_FFT_IMPLEMENTATION
 #include "notorious_fft.h"
+#include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
 #endif
@@ -60,11 +61,25 @@ void ref_exec(notorious_fft_aux *a, int op, real_t *x, real_t *y)
 }
 
 /* Batched run over contiguous transforms; in_n/out_n are reals per
- * transform. nthreads <= 0 uses the OpenMP default. Returns 0 on success. */
+ * transform. nthreads <= 0 uses the OpenMP default. When exec_s is not NULL
+ * it receives the wall time of the transform loop alone (the per-thread aux
+ * creation and release are outside it). Returns 0 on success. */
+static double ref_now(void)
+{
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+#endif
+}
+
 int ref_exec_batch(int kind, int d, const int *Ns, int op, long long batch,
-                   long long in_n, long long out_n, real_t *x, real_t *y, int nthreads)
+                   long long in_n, long long out_n, real_t *x, real_t *y, int nthreads, double *exec_s)
 {
     int err = 0;
+    double t0 = 0.0, t1 = 0.0;
 #ifdef _OPENMP
     if (nthreads <= 0) nthreads = omp_get_max_threads();
 #pragma omp parallel num_threads(nthreads) reduction(| : err)
@@ -72,15 +87,65 @@ int ref_exec_batch(int kind, int d, const int *Ns, int op, long long batch,
     {
         notorious_fft_aux *a = ref_mkaux(kind, d, Ns);
         if (!a) err = 1;
-        else {
 #ifdef _OPENMP
+#pragma omp barrier
+#pragma omp master
+#endif
+        t0 = ref_now();
+#ifdef _OPENMP
+#pragma omp barrier
 #pragma omp for schedule(static)
 #endif
-            for (long long b = 0; b < batch; b++)
-                ref_exec(a, op, x + b * in_n, y + b * out_n);
-            notorious_fft_free_aux(a);
-        }
+        for (long long b = 0; b < batch; b++)
+            if (a) ref_exec(a, op, x + b * in_n, y + b * out_n);
+#ifdef _OPENMP
+#pragma omp master
+#endif
+        t1 = ref_now();
+        if (a) notorious_fft_free_aux(a);
     }
     (void)nthreads;
+    if (exec_s) *exec_s = t1 - t0;
+    return err;
+}
+
+/* A list of 1-D complex transforms of mixed sizes, packed back to back
+ * (transform i of sizes[i] complex starts where transform i-1 ends): the
+ * auxes are made first, then the transforms run OpenMP-parallel over the
+ * list (dynamic schedule); exec_s receives the time of that loop alone.
+ * Returns 0 on success. */
+int ref_exec_list(int count, const int *sizes, int op, real_t *x, real_t *y, int nthreads, double *exec_s)
+{
+    notorious_fft_aux **aux = (notorious_fft_aux **)calloc((size_t)count, sizeof(*aux));
+    long long *off = (long long *)malloc(((size_t)count + 1) * sizeof(long long));
+    int err = 0;
+    if (!aux || !off) {
+        free(aux);
+        free(off);
+        return 1;
+    }
+    off[0] = 0;
+    for (int i = 0; i < count; i++) off[i + 1] = off[i] + 2LL * sizes[i];
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic) reduction(| : err)
+#endif
+    for (int i = 0; i < count; i++) {
+        aux[i] = notorious_fft_mkaux_dft_1d(sizes[i]);
+        if (!aux[i]) err = 1;
+    }
+    double t0 = ref_now();
+    if (!err) {
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic)
+#endif
+        for (int i = 0; i < count; i++) ref_exec(aux[i], op, x + off[i], y + off[i]);
+    }
+    double t1 = ref_now();
+    for (int i = 0; i < count; i++) notorious_fft_free_aux(aux[i]);
+    free(aux);
+    free(off);
+    (void)nthreads;
+    if (exec_s) *exec_s = t1 - t0;
     return err;
 }

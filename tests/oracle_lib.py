@@ -135,7 +135,10 @@ class Ref:
         lib.ref_exec_batch.restype = ctypes.c_int
         lib.ref_exec_batch.argtypes = [ctypes.c_int, ctypes.c_int, _ip, ctypes.c_int,
                                        ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
-                                       rp, rp, ctypes.c_int]
+                                       rp, rp, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        lib.ref_exec_list.restype = ctypes.c_int
+        lib.ref_exec_list.argtypes = [ctypes.c_int, _ip, ctypes.c_int, rp, rp, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_double)]
         self.lib = lib
         self._rp = rp
 
@@ -151,17 +154,36 @@ class Ref:
         return False
 
     def run(self, op, Ns, x, batch=1, nthreads=0):
+        return self.run_timed(op, Ns, x, batch, nthreads)[0]
+
+    def run_list(self, sizes, x, op="dft", nthreads=0):
+        """Mixed-size 1-D complex transforms packed back to back: (output,
+        seconds in the transform loop, auxes made beforehand)."""
+        sizes = list(sizes)
+        x = np.ascontiguousarray(np.asarray(x, dtype=self.np_real).reshape(-1)).copy()
+        assert x.size == 2 * sum(sizes)
+        y = np.zeros_like(x)
+        t = ctypes.c_double(0.0)
+        err = self.lib.ref_exec_list(len(sizes), _ints(sizes), OP[op], x.ctypes.data_as(self._rp),
+                                     y.ctypes.data_as(self._rp), nthreads, ctypes.byref(t))
+        if err:
+            raise ValueError("reference rejected a size of the list")
+        return y, t.value
+
+    def run_timed(self, op, Ns, x, batch=1, nthreads=0):
+        """(output, seconds spent in the transform loop alone)."""
         Ns = list(Ns)
         nin, nout = out_shape(op, Ns)
         x = np.ascontiguousarray(np.asarray(x, dtype=self.np_real).reshape(-1)).copy()
         assert x.size == nin * batch
         y = np.zeros(nout * batch, dtype=self.np_real)
+        t = ctypes.c_double(0.0)
         err = self.lib.ref_exec_batch(KIND[OP_KIND[op]], len(Ns), _ints(Ns), OP[op], batch,
                                       nin, nout, x.ctypes.data_as(self._rp),
-                                      y.ctypes.data_as(self._rp), nthreads)
+                                      y.ctypes.data_as(self._rp), nthreads, ctypes.byref(t))
         if err:
             raise ValueError(f"reference rejected {op} {Ns}")
-        return y
+        return y, t.value
 
 
 def rel_l2(got, ref):
