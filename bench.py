@@ -311,6 +311,21 @@ def cpu_baseline(n=1 << 20, budget_s=10.0):
             "alg_gbs": round(2 * n * 16 * batch / t / 1e9, 3)}
 
 
+def measured_traffic(kernel, n, total):
+    """HBM bytes per launch of `kernel` for this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (calibrated against a plain
+    copy), or None when no summary covers it."""
+    import glob
+    import re
+    tag = f"N2^{int(math.log2(n))}_x{total // n}"
+    for path in glob.glob(os.path.join(ROOT, "profiles", f"*{tag}*pmc_traffic.md")):
+        for line in open(path):
+            m = re.match(r"traffic_bytes_per_launch (\S+) (\d+)", line.strip())
+            if m and m.group(1) == kernel:
+                return int(m.group(2))
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -348,7 +363,8 @@ def main():
     dom = max(kern, key=lambda k: kern[k]["total_ms"])
     dk = kern[dom]
     roofline = {"bound": "hbm", "achieved": round(dk["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dk["gbs"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(dk["gbs"] / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom, args.n, total),
+                "kernel": dom,
                 "avg_launch_us": round(dk["avg_us"], 2), "bytes_per_launch": dk["bytes_per_launch"],
                 "kernels": {k: {"avg_us": round(v["avg_us"], 2), "launches": v["launches"],
                                 "gbs": round(v["gbs"], 1)} for k, v in kern.items()}}
@@ -357,7 +373,7 @@ def main():
     if not args.no_sweep:
         for prec in ("f32", "f64"):
             for lg in range(10, 21):
-                r = run_r2c(pkg, torch, 1 << lg, prec, 1 << 28)
+                r = run_r2c(pkg, torch, 1 << lg, prec, 1 << 28, steps=10, warmup=5)
                 sweep_r2c.append(r)
                 log(f"  sweep r2c {prec} N=2^{lg}: {r['gflops']:.1f} GFLOP/s {100 * r['hbm_frac']:.1f}% HBM "
                     f"rel-L2 {r['rel_l2']:.2e} [{r['plan']}]")
@@ -367,13 +383,17 @@ def main():
         for prec in ("f32", "f64"):
             for lg in range(10, 21):
                 n = 1 << lg
-                r = run_config(pkg, torch, n, prec, total, 5, 2, check=1)
+                r = run_config(pkg, torch, n, prec, total, 10, 5, profile=True, check=1)
+                kern = r["kernels"]
+                dom = max(kern, key=lambda k: kern[k]["total_ms"])
                 sweep.append({"n": n, "prec": prec, "gflops": round(r["gflops"], 1),
                               "alg_gbs": round(r["alg_gbs"], 1), "hbm_frac": round(r["hbm_frac"], 4),
-                              "ms": round(r["ms_per_step"], 4), "rel_l2": float(f"{r['rel_l2']:.3e}"),
-                              "plan": r["plan"]})
+                              "ms": round(r["ms_per_step"], 4), "event_ms": round(r["event_ms_per_step"], 4),
+                              "rel_l2": float(f"{r['rel_l2']:.3e}"), "plan": r["plan"],
+                              "kernel": dom, "kernel_gbs": round(kern[dom]["gbs"], 1),
+                              "kernel_us": round(kern[dom]["avg_us"], 1)})
                 log(f"  sweep {prec} N=2^{lg}: {r['gflops']:.1f} GFLOP/s {100 * r['hbm_frac']:.1f}% HBM "
-                    f"rel-L2 {r['rel_l2']:.2e} [{r['plan']}]")
+                    f"(dominant {dom} {kern[dom]['gbs']:.0f} GB/s) rel-L2 {r['rel_l2']:.2e} [{r['plan']}]")
 
     extras = {}
     if not args.no_sweep:

@@ -162,6 +162,7 @@ struct C2C {
     long long ptimeout = 0;           // wall-clock ticks a dependency wait may take
     int pgrab = 1;                    // queue items taken per access
     int plag = 0;                     // pass-2 lag in transforms (0: half the slots)
+    long long pslots = 1;             // rotating slots at most (Infinity Cache budget)
 
     // Set up tables for length n; returns false with the error set.
     bool init(int n, int prec);

@@ -177,8 +177,25 @@ struct ColArgs {
 
 template <int L, int LPB> __host__ __device__ constexpr int col_threads() { return Sched<L>::NT * LPB; }
 
+// Minimum waves per SIMD the FP32 four-step kernels are compiled for: as many
+// as the workgroups that fit in the LDS can use, capped at 4 (<= 128 VGPRs);
+// without the hint the twiddle pass lands just above 128 VGPRs and loses a
+// quarter of its resident waves.  FP64 kernels keep the compiler's choice.
+template <typename T> __host__ __device__ constexpr int wpe_hint(int threads, long lds_bytes)
+{
+    if (sizeof(T) != 4) return 1;
+    long wg = lds_bytes > 0 ? (160L * 1024) / lds_bytes : 8;
+    if (wg > 8) wg = 8;
+    const long w = wg * (threads / 64) / 4;
+    return w < 1 ? 1 : (w > 4 ? 4 : (int)w);
+}
+
 template <typename T, int L, int LPB, int DIR, int MODE>
-__global__ void __launch_bounds__((col_threads<L, LPB>())) k_col(ColArgs a)
+__global__ void __launch_bounds__((col_threads<L, LPB>()),
+                                  (MODE == COL_BLUE_IN || MODE == COL_BLUE_OUT
+                                       ? 1
+                                       : wpe_hint<T>(col_threads<L, LPB>(), (long)L * LPB * sizeof(Cx<T>))))
+k_col(ColArgs a)
 {
     using S = Sched<L>;
     using C = Cx<T>;
@@ -265,7 +282,9 @@ template <typename T, int L, int LPB> __host__ __device__ constexpr int rowT_lds
 }
 
 template <typename T, int L, int LPB, int DIR>
-__global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT(RowTArgs a)
+__global__ void __launch_bounds__(Sched<L>::NT *LPB,
+                                  (wpe_hint<T>(Sched<L>::NT * LPB, (long)rowT_lds<T, L, LPB>() * sizeof(Cx<T>))))
+k_rowT(RowTArgs a)
 {
     using S = Sched<L>;
     using C = Cx<T>;

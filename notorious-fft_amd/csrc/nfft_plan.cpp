@@ -230,9 +230,13 @@ static int env_int(const char *name, int dflt)
 // Dependency waits of the persistent four-step give up after this long
 // (wall clock); a healthy wait lasts microseconds.
 static const double kPersistTimeoutSec = 2.0;
-// The persistent four-step is opt-in (NFFT_PERSIST=1) while the two-launch
-// path is faster on the measured sizes.
-static const int kPersistDefault = 0;
+// Where the persistent four-step is the default (NFFT_PERSIST=0/1 forces it
+// off/on): the measured crossover on MI355X is FP32 N <= 2^17 and FP64
+// N <= 2^16; above it the two-launch path over whole-batch chunks is faster.
+static bool persist_default(int prec, long long n) { return n <= (prec == F32 ? (1LL << 17) : (1LL << 16)); }
+// Intermediate the persistent kernel keeps in flight (slots), sized to stay
+// in the 256 MB Infinity Cache.
+static const long long kPersistSlotBytes = 128LL << 20;
 // Transforms per persistent launch (bounds the counter block).
 static const long long kPersistCap = 1LL << 15;
 
@@ -255,6 +259,9 @@ static bool persist_setup(C2C &c)
     c.ptimeout = (long long)(kPersistTimeoutSec * 1000.0 * std::max(khz, 1000));
     c.pgrab = std::max(1, env_int("NFFT_PERSIST_GRAB", 1));
     c.plag = std::max(0, env_int("NFFT_PERSIST_LAG", 0));
+    long long mb = env_int("NFFT_PERSIST_MB", 0);
+    const long long bytes_slots = mb > 0 ? (mb << 20) : kPersistSlotBytes;
+    c.pslots = std::max(1LL, bytes_slots / ((long long)c.n * (long long)cx_bytes(c.prec)));
     return true;
 }
 
@@ -275,7 +282,7 @@ bool C2C::init(int n_, int prec_)
     if (is_pow2(n)) {
         algo = ALG_FOUR;
         scratch_per_line = n;
-        if (env_int("NFFT_PERSIST", kPersistDefault) && persist_find(prec, n, env_int("NFFT_PERSIST_VAR", 0), &pcfg)) {
+        if (env_int("NFFT_PERSIST", persist_default(prec, n) ? 1 : 0) && persist_find(prec, n, env_int("NFFT_PERSIST_VAR", 0), &pcfg)) {
             n1 = pcfg.n1;
             n2 = pcfg.n2;
             if (!persist_setup(*this)) return false;
@@ -412,7 +419,7 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
                 a.in_dist = in_dist;
                 a.out_dist = out_dist;
                 a.scratch = scratch;
-                a.slots = (int)std::min(std::min(scratch_lines, cnt), 1LL << 20);
+                a.slots = (int)std::max(1LL, std::min(std::min(scratch_lines, cnt), pslots));
                 a.lag = plag > 0 ? std::min(plag, a.slots - 1) : a.slots / 2;
                 a.ctr = pctr;
                 a.tw_a = tw_a;
@@ -748,12 +755,16 @@ static long long prod(const std::vector<int> &v, size_t b, size_t e)
     return p;
 }
 
-static const long long kChunkBytesDefault = 64LL << 20;
+// Whole batches up to 4 GiB of intermediate run as one launch pair: per-chunk
+// launches cost a fill/drain each (measured: FP64 2^20 x 128 takes 2.27 ms in
+// 64 MiB chunks, 1.68 ms in one).
+static const long long kChunkBytesDefault = 4LL << 30;
 
 // Lines of scratch an engine gets: one chunk of about NFFT_CHUNK_MB of
-// intermediate (default 64 MiB, well inside the 256 MB Infinity Cache).  The
+// intermediate (default 4 GiB, i.e. whole batches of the usual sizes).  The
 // chunked passes process that many lines per launch pair; the persistent
-// four-step uses them as its rotating slots.
+// four-step uses at most NFFT_PERSIST_MB (default 128 MiB) of it as its
+// rotating slots.
 static long long chunk_lines(long long nlines, long long per_line_elems, size_t eb)
 {
     if (per_line_elems <= 0) return 0;
@@ -762,6 +773,14 @@ static long long chunk_lines(long long nlines, long long per_line_elems, size_t 
     long long c = target / (per_line_elems * (long long)eb);
     c = std::max(1LL, std::min(c, nlines));
     return c;
+}
+
+// Scratch elements a complex engine needs for `lines` transforms: the
+// persistent four-step only ever uses its rotating slots.
+static long long engine_elems(const C2C &e, long long lines, size_t cb)
+{
+    if (e.pctr) return std::min(lines, e.pslots) * e.scratch_per_line;
+    return chunk_lines(lines, e.scratch_per_line, cb) * e.scratch_per_line;
 }
 
 static double axis_inv_scale(int norm, int n)
@@ -792,14 +811,14 @@ bool nfft_plan::init(int kind_, int prec_, int d_, const int *dims_, long long b
             dsc += (i ? " | " : "") + ax[i].describe();
         }
         const long long lines = batch * P / last;
-        eng = chunk_lines(lines, ax[d - 1].scratch_per_line, cb) * ax[d - 1].scratch_per_line;
+        eng = engine_elems(ax[d - 1], lines, cb);
         for (int i = 0; i + 1 < d; i++) {
             const C2C &e = ax[i];
             const bool direct = dims[i] == 1 || (e.algo == ALG_LINE && dims[i] <= col_max(prec));
             if (!direct) {
                 full = std::max(full, batch * P);
                 const long long lines_i = batch * P / dims[i];
-                eng = std::max(eng, chunk_lines(lines_i, e.scratch_per_line, cb) * e.scratch_per_line);
+                eng = std::max(eng, engine_elems(e, lines_i, cb));
             }
         }
         work_bytes = (size_t)(full + eng) * cb;
@@ -823,7 +842,7 @@ bool nfft_plan::init(int kind_, int prec_, int d_, const int *dims_, long long b
             const bool direct = dims[i] == 1 || (e.algo == ALG_LINE && dims[i] <= col_max(prec));
             if (!direct) {
                 const long long lines_i = batch * Pl * H / dims[i];
-                eng = std::max(eng, batch * Pl * H + chunk_lines(lines_i, e.scratch_per_line, cb) * e.scratch_per_line);
+                eng = std::max(eng, batch * Pl * H + engine_elems(e, lines_i, cb));
             }
         }
         full = cfull;

@@ -183,6 +183,7 @@ def test_four_step_small_scratch(pkg, dev, prec, n, batch, persist, monkeypatch)
     rotating slot (every pass-1 tile waits for the previous transform's pass
     2), the two-launch path with many chunks; in place too."""
     monkeypatch.setenv("NFFT_CHUNK_MB", "1")
+    monkeypatch.setenv("NFFT_PERSIST_MB", "1")
     monkeypatch.setenv("NFFT_PERSIST", persist)
     x = rounded(ol.uniform(7, 2 * n * batch), prec)
     ref = ol.run_oracle("dft", [n], x, batch)
@@ -219,7 +220,7 @@ def test_persist_many_transforms(pkg, dev, monkeypatch):
     n, batch = 1 << 15, 300
     x = rounded(ol.uniform(9, 2 * n * batch), "f32")
     ref = ol.run_oracle("dft", [n], x, batch)
-    monkeypatch.setenv("NFFT_CHUNK_MB", "2")
+    monkeypatch.setenv("NFFT_PERSIST_MB", "2")
     monkeypatch.setenv("NFFT_PERSIST", "1")
     got = run_plan(pkg, dev, "dft", [n], x, batch, "f32")
     assert ol.rel_l2(got, ref) < tol_for("f32", n)
