@@ -238,10 +238,39 @@ template <int DIR, typename C> __device__ __forceinline__ C ctw64(C a, double2 w
 }
 
 // Radix-16 stage twiddles w^r (r = 1..15) from the base powers
-// b = [w, w^2, w^4, w^8]: 11 FP64 products, each power at most 3 products
-// away from correctly rounded bases.
+// b = [w, w^2, w^4, w^8] (FP64 tables): 11 products, each power at most 3
+// products away from correctly rounded bases.  FP64 lines multiply in FP64;
+// FP32 lines round the four bases once and form the powers in FP32 (a few
+// FP32 ulps per twiddle, at half the cost of FP64 products and without the
+// per-twiddle FP64 -> FP32 conversions).
+template <int DIR> __device__ __forceinline__ void apply_stage_tw32(Cx<float> *a, const double2 (&b)[4])
+{
+    const Cx<float> w1 = to_cx<float>(b[0]), w2 = to_cx<float>(b[1]), w4 = to_cx<float>(b[2]),
+                    w8 = to_cx<float>(b[3]);
+    const Cx<float> w3 = cmul(w2, w1), w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
+    a[1] = ctw<DIR>(a[1], w1);
+    a[2] = ctw<DIR>(a[2], w2);
+    a[3] = ctw<DIR>(a[3], w3);
+    a[4] = ctw<DIR>(a[4], w4);
+    a[5] = ctw<DIR>(a[5], w5);
+    a[6] = ctw<DIR>(a[6], w6);
+    a[7] = ctw<DIR>(a[7], w7);
+    a[8] = ctw<DIR>(a[8], w8);
+    a[9] = ctw<DIR>(a[9], cmul(w8, w1));
+    a[10] = ctw<DIR>(a[10], cmul(w8, w2));
+    a[11] = ctw<DIR>(a[11], cmul(w8, w3));
+    a[12] = ctw<DIR>(a[12], cmul(w8, w4));
+    a[13] = ctw<DIR>(a[13], cmul(w8, w5));
+    a[14] = ctw<DIR>(a[14], cmul(w8, w6));
+    a[15] = ctw<DIR>(a[15], cmul(w8, w7));
+}
+
 template <int DIR, typename C> __device__ __forceinline__ void apply_stage_tw(C *a, const double2 (&b)[4])
 {
+    if constexpr (sizeof(a[0].x) == 4) {
+        apply_stage_tw32<DIR>(a, b);
+        return;
+    }
     // ordered so that at most two FP64 temporaries are live at a time
     a[1] = ctw64<DIR>(a[1], b[0]);
     a[2] = ctw64<DIR>(a[2], b[1]);
@@ -343,16 +372,33 @@ __device__ __forceinline__ double2 tw2_get(const Tw2 &t, uint32_t e)
     return r;
 }
 // reg[q] *= w^{base_e + step_e * q} (conj for DIR > 0), q = 0..R-1: two
-// table lookups, then an FP64 recurrence (error ~R ulps of FP64).
+// table lookups, then a recurrence.  FP64 lines recur in FP64 (error ~R ulps
+// of FP64).  FP32 lines form w^base, w^(base+step) and w^(2 step) in FP64,
+// round them once and run two interleaved FP32 chains of R/2 products each
+// (error ~R/2 FP32 ulps on the last twiddles).
 template <int DIR, typename C, int R>
 __device__ __forceinline__ void apply_tw2_series(C (&reg)[R], const Tw2 &t, uint32_t base_e, uint32_t step_e)
 {
     double2 w = tw2_get(t, base_e);
     const double2 st = tw2_get(t, step_e);
+    if constexpr (sizeof(reg[0].x) == 4 && R % 2 == 0) {
+        Cx<float> e = to_cx<float>(w), o = to_cx<float>(dmul(w, st));
+        const Cx<float> st2 = to_cx<float>(dmul(st, st));
 #pragma unroll
-    for (int q = 0; q < R; q++) {
-        reg[q] = ctw64<DIR>(reg[q], w);
-        if (q + 1 < R) w = dmul(w, st);
+        for (int q = 0; q < R; q += 2) {
+            reg[q] = ctw<DIR>(reg[q], e);
+            reg[q + 1] = ctw<DIR>(reg[q + 1], o);
+            if (q + 2 < R) {
+                e = cmul(e, st2);
+                o = cmul(o, st2);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            reg[q] = ctw64<DIR>(reg[q], w);
+            if (q + 1 < R) w = dmul(w, st);
+        }
     }
 }
 

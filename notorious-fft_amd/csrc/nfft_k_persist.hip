@@ -61,7 +61,9 @@ const Entry kShapes[] = {
     NFFT_PERSIST(float, F32, 256, 256, 32, 32, 1),
     NFFT_PERSIST(float, F32, 512, 256, 16, 32, 1),
     NFFT_PERSIST(float, F32, 512, 512, 16, 16, 1),
+    NFFT_PERSIST(float, F32, 512, 512, 16, 16, 2),
     NFFT_PERSIST(float, F32, 1024, 512, 8, 16, 1),
+    NFFT_PERSIST(float, F32, 1024, 512, 8, 16, 2),
     NFFT_PERSIST(float, F32, 1024, 1024, 8, 8, 1),
     NFFT_PERSIST(float, F32, 1024, 1024, 16, 16, 1),
     NFFT_PERSIST(float, F32, 512, 2048, 16, 4, 1),
