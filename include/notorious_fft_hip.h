@@ -113,6 +113,12 @@ int nfft_ragged_execute(nfft_ragged *r, int inverse, const void *in, void *out, 
 int nfft_ragged_destroy(nfft_ragged *r);
 long long nfft_ragged_total(const nfft_ragged *r);
 
+/* Synchronises `stream` and reports whether an earlier execute of `r` hit a
+ * device-side fault: the tiles of the long transforms wait for each other
+ * with bounded waits that raise a flag instead of spinning forever. Returns
+ * NFFT_OK, or NFFT_E_HIP with the message set (outputs then undefined). */
+int nfft_ragged_check(nfft_ragged *r, void *stream);
+
 /* Kernel timer.  nfft_prof_enable(1, capacity) clears the records and
  * brackets every subsequent kernel launch of the library with HIP events on
  * the launch stream (up to `capacity` launches); (0, 0) stops recording.

@@ -107,6 +107,8 @@ def core():
             lib.nfft_ragged_destroy.argtypes = [_vp]
             lib.nfft_ragged_total.restype = _ll
             lib.nfft_ragged_total.argtypes = [_vp]
+            lib.nfft_ragged_check.restype = ctypes.c_int
+            lib.nfft_ragged_check.argtypes = [_vp, _vp]
         _core = lib
     return _core
 

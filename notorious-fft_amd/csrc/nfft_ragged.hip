@@ -6,11 +6,17 @@
 // the whole ragged batch is one launch:
 //   - the host sorts the transforms into work items: a group of up to LPB
 //     same-size short transforms (power of two <= 4096, or Bluestein with
-//     M <= 4096) processed as LDS-resident lines, or one long transform
-//     (power of two up to 65536: four-step; Bluestein up to M = 131072:
-//     three passes) processed by a single workgroup through its own scratch
-//     slice, so no work item ever waits for another workgroup;
-//   - items are ordered longest-first and handed out by an atomic counter;
+//     M <= 4096) processed as LDS-resident lines, or one tile of a long
+//     transform (power of two up to 65536: four-step, 2 phases; Bluestein
+//     up to M = 131072: 3 phases of column / row / column tiles) whose
+//     intermediate lives in one of a set of rotating scratch slots;
+//   - the long transforms' tiles come first, in a software-pipelined order
+//     (phase 1 of transform s, phase 2 of s - D, phase 3 of s - 2D), then
+//     the short groups longest-first; an atomic counter hands items out;
+//   - a tile waits (bounded, with an error flag) for the previous phase of
+//     its transform, a phase-1 tile for the last phase of its slot's
+//     previous user; both were handed out earlier, so the queue cannot
+//     deadlock;
 //   - the graph holds [counter reset kernel, persistent kernel]; there is no
 //     memset node.
 // Inverse transforms follow the reference's normalisation: unnormalised for
@@ -36,14 +42,23 @@ namespace nfft {
 constexpr int RG_THREADS = 256;
 enum RagType { RG_LINE = 0, RG_BLUE1 = 1, RG_FOUR = 2, RG_BLUE3 = 3 };
 
+// A work item: a group of short transforms (LINE, BLUE1), or one tile of a
+// long transform (FOUR: phase 1 column tiles, phase 2 row tiles; BLUE3:
+// phases 1-3 = column, row, column tiles), covering column/row groups
+// [g0, g1) of that phase.
 struct RagItem {
     int type;
-    int L, L2;   // LINE: n; BLUE1: M; FOUR: n1 x n2; BLUE3: m1 x m2
+    int L, L2;     // LINE: n; BLUE1: M; FOUR: n1 x n2; BLUE3: m1 x m2
     int n;
-    int count;   // transforms in this item
-    int first;   // first entry in the id list
-    int tab;     // table descriptor
-    int pad_;
+    int count;     // transforms in this item (tiles: 1)
+    int first;     // first entry in the id list
+    int tab;       // table descriptor
+    int phase;     // tiles: 1..3
+    int g0, g1;    // tiles: group range
+    int slot;      // tiles: intermediate slot
+    int done;      // tiles: counter bumped on completion
+    int wait;      // tiles: counter to wait on (-1: none)
+    int need;      // tiles: count it must reach
 };
 
 struct RagTab {
@@ -60,9 +75,10 @@ struct RagArgs {
     const int *ids;
     const long long *offs;
     const RagTab *tabs;
-    int *counter;
-    void *scratch;
-    long long scratch_per_block;
+    unsigned *ctr;          // [0] queue head, [1] error, [2] sticky error, [3 + k] tile counters
+    void *scratch;          // slots x slot_elems intermediate elements
+    long long slot_elems;
+    long long timeout;      // wall-clock ticks a dependency wait may take
 };
 
 template <int L> constexpr int rg_lines() { return RG_THREADS / Sched<L>::NT; }
@@ -150,27 +166,35 @@ __device__ void rg_line(const RagArgs &a, const RagItem &it, Cx<T> *lds, long lo
 // ---------------------------------------------------------------------------
 // one long transform per workgroup
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void rg_fence_barrier()
+// Scratch accesses of tiles go through agent-scope (sc1) loads and stores,
+// coherent across XCDs without cache maintenance (see nfft_kernels_persist.h).
+template <bool COH, typename C> __device__ __forceinline__ C rg_ld(const C *p)
 {
-    __threadfence();
-    __syncthreads();
-    __threadfence();
+    if constexpr (COH) return coh_load(p);
+    else return *p;
+}
+template <bool COH, typename C> __device__ __forceinline__ void rg_st(C *p, C v)
+{
+    if constexpr (COH) coh_store(p, v);
+    else *p = v;
 }
 
-// column pass over an L1 x L2 matrix (row-major, L2 columns), all columns:
-// MODE 0: src -> FFT_L1 -> * tw2 (c*p, DIR) -> dst
-// MODE 1: x[n] * chirp (n < nv, conj if cj) -> FFT -> * tw2 -> dst (Bluestein in)
-// MODE 2: src -> IFFT -> y[n] = conj?(v * chirp[n]) * sc, n < nv (Bluestein out)
+// column pass over an L1 x L2 matrix (row-major, L2 columns), column groups
+// [g0, g1) of LPB columns each:
+// MODE 0: src -> FFT_L1 -> * tw2 (c*p, DIR) -> dst (scratch)
+// MODE 1: x[n] * chirp (n < nv, conj if cj) -> FFT -> * tw2 -> dst (scratch; Bluestein in)
+// MODE 2: src (scratch) -> IFFT -> y[n] = conj?(v * chirp[n]) * sc, n < nv (Bluestein out)
 template <typename T, int L1, int L2, int DIR, int MODE>
-__device__ void rg_cols(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, int nv, int cj, T sc, Cx<T> *lds)
+__device__ void rg_cols(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, int nv, int cj, T sc, Cx<T> *lds, int g0,
+                        int g1)
 {
     using S = Sched<L1>;
     using C = Cx<T>;
     constexpr int LPB = RG_THREADS / S::NT;
     static_assert(L2 % LPB == 0, "column tiles must cover the columns");
     const int b = threadIdx.x % LPB, j = threadIdx.x / LPB;
-    for (int c0 = 0; c0 < L2; c0 += LPB) {
-        const int c = c0 + b;
+    for (int g = g0; g < g1; g++) {
+        const int c = g * LPB + b;
         C reg[S::R];
 #pragma unroll
         for (int q = 0; q < S::R; q++) {
@@ -185,52 +209,52 @@ __device__ void rg_cols(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, int nv, 
                 }
                 reg[q] = v;
             } else {
-                reg[q] = src[n];
+                reg[q] = rg_ld<MODE == 2>(src + n);
             }
         }
         LayCol<LPB> lay{b};
         line_fft<T, L1, DIR>(reg, lds, lay, j, (const double2 *)tb.tw_a);
+        if constexpr (MODE == 2) {
 #pragma unroll
-        for (int q = 0; q < S::R; q++) {
-            const int p = j + S::NT * q;
-            const int n = p * L2 + c;
-            if constexpr (MODE == 2) {
+            for (int q = 0; q < S::R; q++) {
+                const int p = j + S::NT * q;
+                const int n = p * L2 + c;
                 if (n < nv) {
                     C v = cmul(reg[q], ((const C *)tb.chirp)[n]);
                     if (cj) v = cconj(v);
                     dst[n] = cscale(v, sc);
                 }
             }
-        }
-        if constexpr (MODE != 2) {
+        } else {
             apply_tw2_series<DIR>(reg, tb.tw2, (uint32_t)c * (uint32_t)j, (uint32_t)c * (uint32_t)S::NT);
 #pragma unroll
-            for (int q = 0; q < S::R; q++) dst[(j + S::NT * q) * L2 + c] = reg[q];
+            for (int q = 0; q < S::R; q++) rg_st<true>(dst + (j + S::NT * q) * L2 + c, reg[q]);
         }
     }
 }
 
-// row pass: rows of L2 points, L1 rows.
-// MODE 0: FFT_L2 (DIR), store transposed to dst[r + L1*k] (four-step pass 2)
-// MODE 1: FFT, * H[r][k], IFFT, * conj(tw2(r*n2)), store in place (Bluestein middle)
+// row pass: rows of L2 points, row groups [g0, g1) of LPB rows each.
+// MODE 0: scratch -> FFT_L2 (DIR) -> transposed store dst[r + L1*k] (four-step pass 2)
+// MODE 1: scratch -> FFT, * H[r][k], IFFT, * conj(tw2(r*n2)) -> scratch in place (Bluestein middle)
 template <typename T, int L1, int L2, int DIR, int MODE>
-__device__ void rg_rows(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, Cx<T> *lds)
+__device__ void rg_rows(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, Cx<T> *lds, int g0, int g1)
 {
     using S = Sched<L2>;
     using C = Cx<T>;
     constexpr int LPB = RG_THREADS / S::NT;
     constexpr int PADL = lpadlen<T, L2>();
     const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
-    for (int r0 = 0; r0 < L1; r0 += LPB) {
+    for (int g = g0; g < g1; g++) {
+        const int r0 = g * LPB;
         const int r = r0 + b;
         C reg[S::R];
         if constexpr (S::NT >= 64) {
 #pragma unroll
-            for (int q = 0; q < S::R; q++) reg[q] = src[(long long)r * L2 + j + S::NT * q];
+            for (int q = 0; q < S::R; q++) reg[q] = rg_ld<true>(src + (long long)r * L2 + j + S::NT * q);
         } else {
             for (int idx = threadIdx.x; idx < LPB * L2; idx += RG_THREADS) {
                 const int bb = idx / L2, p = idx % L2;
-                lds[bb * PADL + lpad<T>(p)] = src[(long long)(r0 + bb) * L2 + p];
+                lds[bb * PADL + lpad<T>(p)] = rg_ld<true>(src + (long long)(r0 + bb) * L2 + p);
             }
             __syncthreads();
 #pragma unroll
@@ -257,14 +281,14 @@ __device__ void rg_rows(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, Cx<T> *l
             apply_tw2_series<1>(reg, tb.tw2, (uint32_t)r * (uint32_t)j, (uint32_t)r * (uint32_t)S::NT);
             if constexpr (S::NT >= 64) {
 #pragma unroll
-                for (int q = 0; q < S::R; q++) dst[(long long)r * L2 + j + S::NT * q] = reg[q];
+                for (int q = 0; q < S::R; q++) rg_st<true>(dst + (long long)r * L2 + j + S::NT * q, reg[q]);
             } else {
 #pragma unroll
                 for (int q = 0; q < S::R; q++) lds[b * PADL + lpad<T>(j + S::NT * q)] = reg[q];
                 __syncthreads();
                 for (int idx = threadIdx.x; idx < LPB * L2; idx += RG_THREADS) {
                     const int bb = idx / L2, p = idx % L2;
-                    dst[(long long)(r0 + bb) * L2 + p] = lds[bb * PADL + lpad<T>(p)];
+                    rg_st<true>(dst + (long long)(r0 + bb) * L2 + p, lds[bb * PADL + lpad<T>(p)]);
                 }
                 __syncthreads();
             }
@@ -272,34 +296,34 @@ __device__ void rg_rows(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, Cx<T> *l
     }
 }
 
+// one tile of a long power-of-two transform (four-step N1 x N2)
 template <typename T, int N1, int N2, int DIR>
 __device__ void rg_four(const RagArgs &a, const RagItem &it, Cx<T> *lds)
 {
     using C = Cx<T>;
     const RagTab tb = a.tabs[it.tab];
     const long long base = a.offs[a.ids[it.first]];
-    C *scr = (C *)a.scratch + (long long)blockIdx.x * a.scratch_per_block;
-    rg_cols<T, N1, N2, DIR, 0>((const C *)a.in + base, scr, tb, 0, 0, (T)1, lds);
-    rg_fence_barrier();
-    rg_rows<T, N1, N2, DIR, 0>(scr, (C *)a.out + base, tb, lds);
-    rg_fence_barrier();
+    C *scr = (C *)a.scratch + (long long)it.slot * a.slot_elems;
+    if (it.phase == 1) rg_cols<T, N1, N2, DIR, 0>((const C *)a.in + base, scr, tb, 0, 0, (T)1, lds, it.g0, it.g1);
+    else rg_rows<T, N1, N2, DIR, 0>(scr, (C *)a.out + base, tb, lds, it.g0, it.g1);
 }
 
+// one tile of a long Bluestein transform (three passes over M1 x M2)
 template <typename T, int M1, int M2, int DIR>
 __device__ void rg_blue3(const RagArgs &a, const RagItem &it, Cx<T> *lds)
 {
     using C = Cx<T>;
     const RagTab tb = a.tabs[it.tab];
     const long long base = a.offs[a.ids[it.first]];
-    C *scr = (C *)a.scratch + (long long)blockIdx.x * a.scratch_per_block;
+    C *scr = (C *)a.scratch + (long long)it.slot * a.slot_elems;
     const int cj = DIR > 0 ? 1 : 0;
     const T sc = DIR > 0 ? (T)1 / (T)it.n : (T)1;
-    rg_cols<T, M1, M2, -1, 1>((const C *)a.in + base, scr, tb, it.n, cj, (T)1, lds);
-    rg_fence_barrier();
-    rg_rows<T, M1, M2, -1, 1>(scr, scr, tb, lds);
-    rg_fence_barrier();
-    rg_cols<T, M1, M2, 1, 2>(scr, (C *)a.out + base, tb, it.n, cj, sc, lds);
-    rg_fence_barrier();
+    if (it.phase == 1)
+        rg_cols<T, M1, M2, -1, 1>((const C *)a.in + base, scr, tb, it.n, cj, (T)1, lds, it.g0, it.g1);
+    else if (it.phase == 2)
+        rg_rows<T, M1, M2, -1, 1>(scr, scr, tb, lds, it.g0, it.g1);
+    else
+        rg_cols<T, M1, M2, 1, 2>(scr, (C *)a.out + base, tb, it.n, cj, sc, lds, it.g0, it.g1);
 }
 
 // ---------------------------------------------------------------------------
@@ -312,12 +336,36 @@ __global__ void __launch_bounds__(RG_THREADS) k_ragged(RagArgs a)
     Cx<T> *lds = (Cx<T> *)smem;
     __shared__ long long bases[RG_THREADS];
     __shared__ int cur;
-    for (;;) {
-        if (threadIdx.x == 0) cur = atomicAdd(a.counter, 1);
+    unsigned *head = a.ctr, *err = a.ctr + 1, *tiles = a.ctr + 3;
+    // Lane 0 of wave 0 publishes the finished tile and fetches the next item
+    // in one region behind a wave-uniform branch (a second lane-divergent
+    // region between this loop's barriers lets the compiler reorder lane 0's
+    // work past the next barrier).  Tiles wait only for items earlier in the
+    // queue, held by running workgroups, so the queue cannot deadlock; each
+    // wait is bounded and raises the error flag instead of spinning.
+    const bool wave0 = __builtin_amdgcn_readfirstlane(threadIdx.x) == 0;
+    int done = -1;  // lane 0: counter of the tile just finished
+    for (long long iter = 0;; iter++) {
+        if (iter > a.nitems) {  // cannot happen: a workgroup takes at most nitems items
+            if (threadIdx.x == 0) atomicOr(err, 2u);
+            break;
+        }
+        if (wave0) {
+            if (threadIdx.x == 0) {
+                if (done >= 0) __hip_atomic_fetch_add(tiles + done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const int i = (int)atomicAdd(head, 1u);
+                done = -1;
+                if (i < a.nitems) {
+                    const RagItem &q = a.items[i];
+                    if (q.wait >= 0) persist_wait(tiles + q.wait, (unsigned)q.need, err, a.timeout);
+                    done = q.done;
+                }
+                cur = i;
+            }
+        }
         __syncthreads();
-        const int i = cur;
-        __syncthreads();
-        if (i >= a.nitems) break;  // every workgroup exits once the queue is drained
+        const int i = __builtin_amdgcn_readfirstlane(cur);
+        if (i >= a.nitems) break;  // uniform: the queue is drained
         const RagItem it = a.items[i];
         switch (it.type) {
         case RG_LINE:
@@ -354,12 +402,13 @@ __global__ void __launch_bounds__(RG_THREADS) k_ragged(RagArgs a)
             }
             break;
         }
+        // tiles that wrote the intermediate: each wave's stores are complete
+        // (acknowledged at agent scope) before the barrier that precedes
+        // lane 0's counter increment
+        if ((it.type == RG_FOUR && it.phase == 1) || (it.type == RG_BLUE3 && it.phase < 3))
+            __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
     }
-}
-
-__global__ void k_reset_counter(int *c)
-{
-    if (threadIdx.x == 0 && blockIdx.x == 0) *c = 0;
 }
 
 // LDS bytes of the largest path: 256 threads x 16 points (+ padding / the
@@ -385,7 +434,12 @@ int ilog2i(long long n)
     while ((1LL << l) < n) l++;
     return l;
 }
-// splits used inside one workgroup (see the kernel's dispatch table)
+int env_int_rg(const char *name, int dflt)
+{
+    const char *v = getenv(name);
+    return v && *v ? atoi(v) : dflt;
+}
+// splits of the long transforms (see the kernel's dispatch table)
 bool four_split(int n, int *n1, int *n2)
 {
     switch (n) {
@@ -416,9 +470,12 @@ struct nfft_ragged {
     int count = 0;
     long long total = 0;
     std::vector<C2C> engines;  // per distinct size (tables)
-    void *d_items = nullptr, *d_ids = nullptr, *d_offs = nullptr, *d_tabs = nullptr, *d_counter = nullptr;
-    void *scratch = nullptr;
-    long long scratch_per_block = 0;
+    void *d_items = nullptr, *d_ids = nullptr, *d_offs = nullptr, *d_tabs = nullptr;
+    unsigned *ctr = nullptr;   // queue head, error flags, tile counters
+    long long ctr_words = 0;
+    void *scratch = nullptr;   // rotating intermediate slots of the long transforms
+    long long slot_elems = 0;
+    long long timeout = 0;
     int nitems = 0;
     int grid = 0;
     size_t lds = 0;
@@ -430,9 +487,9 @@ struct nfft_ragged {
 static int launch_ragged(nfft_ragged *r, int inverse, const void *in, void *out, hipStream_t s)
 {
     RagArgs a{in, out, (const RagItem *)r->d_items, r->nitems, (const int *)r->d_ids,
-              (const long long *)r->d_offs, (const RagTab *)r->d_tabs, (int *)r->d_counter, r->scratch,
-              r->scratch_per_block};
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, (int *)r->d_counter);
+              (const long long *)r->d_offs, (const RagTab *)r->d_tabs, r->ctr, r->scratch,
+              r->slot_elems, r->timeout};
+    if (launch_persist_reset(r->ctr, r->ctr_words, s) != hipSuccess) return NFFT_E_HIP;
     if (r->prec == F32) {
         if (inverse) hipLaunchKernelGGL((k_ragged<float, 1>), dim3(r->grid), dim3(RG_THREADS), r->lds, s, a);
         else hipLaunchKernelGGL((k_ragged<float, -1>), dim3(r->grid), dim3(RG_THREADS), r->lds, s, a);
@@ -451,7 +508,7 @@ static void ragged_free(nfft_ragged *r)
     }
     for (auto &e : r->engines) e.release();
     for (void *p : r->owned) (void)hipFree(p);
-    void *ptrs[] = {r->d_items, r->d_ids, r->d_offs, r->d_tabs, r->d_counter, r->scratch};
+    void *ptrs[] = {r->d_items, r->d_ids, r->d_offs, r->d_tabs, r->ctr, r->scratch};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     if (r->cap) (void)hipStreamDestroy(r->cap);
@@ -513,7 +570,6 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
     std::vector<long long> offs(count);
     std::map<int, int> size_tab;  // size -> table index
     std::vector<RagTab> tabs;
-    long long max_scratch = 0;
     for (int i = 0; i < count; i++) {
         const int n = sizes[i];
         if (n < 1) {
@@ -544,7 +600,6 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
             t.tw_a = stage_table(prec, n1);
             t.tw_b = stage_table(prec, n2);
             if (!t.tw_a || !t.tw_b || !tw2_table(n, &t.tw2)) { ragged_free(r); return NFFT_E_HIP; }
-            max_scratch = std::max(max_scratch, (long long)n);
         } else {
             // Bluestein: the engine builds chirp and H in the layout it uses
             // for M <= line limit (natural) or M1 x M2 (permuted); the ragged
@@ -588,65 +643,150 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
                 if (st) (void)hipStreamDestroy(st);
                 if (!ok) { set_error("ragged Bluestein table setup failed"); ragged_free(r); return NFFT_E_HIP; }
                 t.H = Hn;
-                max_scratch = std::max(max_scratch, m);
             }
             r->engines.push_back(e);  // keeps chirp/H alive
         }
         size_tab[n] = (int)tabs.size();
         tabs.push_back(t);
     }
-    // work items
+    // work items: groups of short transforms, and tiles of the long ones
+    // (FOUR, BLUE3), each long transform split into phases of column/row
+    // tiles that many workgroups share
+    struct Long {
+        int id, type, L, L2, n, tab, phases, groups, tiles;
+        long long elems;
+        double cost;
+    };
+    std::vector<Long> longs;
     std::map<std::pair<int, int>, std::vector<int>> groups;  // (type, size) -> ids
     for (int i = 0; i < count; i++) {
         const int n = sizes[i];
-        int type;
-        if (is_pow2(n)) type = n <= 4096 ? RG_LINE : RG_FOUR;
-        else {
-            long long m = 1;
-            while (m < 2LL * n - 1) m <<= 1;
-            type = m <= 4096 ? RG_BLUE1 : RG_BLUE3;
+        long long m = 1;
+        while (m < 2LL * n - 1) m <<= 1;
+        if (is_pow2(n) && n > 4096) {
+            Long l{};
+            l.id = i;
+            l.type = RG_FOUR;
+            four_split(n, &l.L, &l.L2);
+            l.n = n;
+            l.tab = size_tab[n];
+            l.phases = 2;
+            l.groups = n / (RG_THREADS * 16);
+            l.elems = n;
+            l.cost = 4.0 * n;
+            longs.push_back(l);
+        } else if (!is_pow2(n) && m > 4096) {
+            Long l{};
+            l.id = i;
+            l.type = RG_BLUE3;
+            four_split((int)m, &l.L, &l.L2);
+            l.n = n;
+            l.tab = size_tab[n];
+            l.phases = 3;
+            l.groups = (int)(m / (RG_THREADS * 16));
+            l.elems = m;
+            l.cost = 6.0 * m;
+            longs.push_back(l);
+        } else {
+            groups[{is_pow2(n) ? RG_LINE : RG_BLUE1, n}].push_back(i);
         }
-        groups[{type, n}].push_back(i);
     }
-    std::vector<RagItem> items;
+    std::stable_sort(longs.begin(), longs.end(), [](const Long &a, const Long &b) { return a.cost > b.cost; });
+    const int gpt = std::max(1, env_int_rg("NFFT_RAGGED_GPT", 4));  // column/row groups per tile (measured best)
+    long long slot_elems = 0;
+    for (Long &l : longs) {
+        l.tiles = (l.groups + gpt - 1) / gpt;
+        slot_elems = std::max(slot_elems, l.elems);
+    }
+    const long long B = (long long)longs.size();
+    // rotating slots within a memory budget; the pipeline lag D (in long
+    // transforms) must stay below slots/2 so that a slot's next user waits
+    // for work handed out well before it
+    long long slots = 0;
+    int lag = 0;
+    if (B > 0) {
+        const long long budget = (long long)std::max(16, env_int_rg("NFFT_RAGGED_MB", 512)) << 20;
+        slots = std::max(1LL, std::min(B, budget / (slot_elems * (long long)cb)));
+        lag = std::min<long long>(std::max(0, env_int_rg("NFFT_RAGGED_LAG", 16)), (slots - 1) / 2);
+    }
+    std::vector<RagItem> sorted;
     std::vector<int> ids;
+    std::vector<int> first(B);
+    for (long long b = 0; b < B; b++) {
+        first[b] = (int)ids.size();
+        ids.push_back(longs[b].id);
+    }
+    auto emit = [&](long long b, int phase) {
+        const Long &l = longs[b];
+        for (int t = 0; t < l.tiles; t++) {
+            RagItem it{};
+            it.type = l.type;
+            it.L = l.L;
+            it.L2 = l.L2;
+            it.n = l.n;
+            it.count = 1;
+            it.first = first[b];
+            it.tab = l.tab;
+            it.phase = phase;
+            it.g0 = t * gpt;
+            it.g1 = std::min(l.groups, (t + 1) * gpt);
+            it.slot = (int)(b % slots);
+            it.done = (int)(3 * b + phase - 1);
+            if (phase > 1) {
+                it.wait = (int)(3 * b + phase - 2);
+                it.need = l.tiles;
+            } else if (b >= slots) {
+                const Long &prev = longs[b - slots];
+                it.wait = (int)(3 * (b - slots) + prev.phases - 1);
+                it.need = prev.tiles;
+            } else {
+                it.wait = -1;
+            }
+            sorted.push_back(it);
+        }
+    };
+    for (long long st = 0; st < B + 2LL * lag; st++) {
+        if (st < B) emit(st, 1);
+        if (st - lag >= 0 && st - lag < B) emit(st - lag, 2);
+        if (st - 2 * lag >= 0 && st - 2 * lag < B && longs[st - 2 * lag].phases == 3) emit(st - 2 * lag, 3);
+    }
+    std::vector<RagItem> short_items;
     std::vector<double> cost;
     for (auto &kv : groups) {
         const int type = kv.first.first, n = kv.first.second;
         const std::vector<int> &v = kv.second;
         long long m = 1;
         while (m < 2LL * n - 1) m <<= 1;
-        int L = n, L2 = 0, per = 1;
-        double c1 = 0;
-        if (type == RG_LINE) { L = n; per = RG_THREADS / std::max(1, L / 16); c1 = 2.0 * n; }
-        else if (type == RG_BLUE1) { L = (int)m; per = RG_THREADS / std::max(1, L / 16); c1 = 2.0 * n + 0.5 * m; }
-        else if (type == RG_FOUR) { four_split(n, &L, &L2); c1 = 4.0 * n; }
-        else { four_split((int)m, &L, &L2); c1 = 6.0 * m; }
-        if (L < 16 && (type == RG_LINE || type == RG_BLUE1)) per = RG_THREADS;
+        const int L = type == RG_LINE ? n : (int)m;
+        int per = RG_THREADS / std::max(1, L / 16);
+        if (L < 16) per = RG_THREADS;
+        const double c1 = type == RG_LINE ? 2.0 * n : 2.0 * n + 0.5 * m;
         for (size_t k = 0; k < v.size(); k += per) {
             RagItem it{};
             it.type = type;
             it.L = L;
-            it.L2 = L2;
             it.n = n;
             it.count = (int)std::min<size_t>(per, v.size() - k);
             it.first = (int)ids.size();
             it.tab = size_tab[n];
+            it.done = -1;
+            it.wait = -1;
             for (size_t q = k; q < k + it.count; q++) ids.push_back(v[q]);
-            items.push_back(it);
+            short_items.push_back(it);
             cost.push_back(c1 * it.count);
         }
     }
-    // longest first
-    std::vector<int> order(items.size());
+    // short groups after the long transforms' tiles, longest first
+    std::vector<int> order(short_items.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<RagItem> sorted;
-    for (int o : order) sorted.push_back(items[o]);
+    for (int o : order) sorted.push_back(short_items[o]);
     r->nitems = (int)sorted.size();
-    int dev = 0, ncu = 256;
+    int dev = 0, ncu = 256, khz = 0;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+    r->timeout = (long long)(2.0 * 1000.0 * std::max(khz, 1000));
     r->lds = rg_lds_bytes(prec);
     {
         const int lds = (int)r->lds;
@@ -666,15 +806,15 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
     int per_cu = tune_lookup(key);
     if (per_cu <= 0) per_cu = prec == F32 ? 4 : 2;
     r->grid = std::max(1, std::min(r->nitems, ncu * per_cu));
-    r->scratch_per_block = max_scratch;
-    const int max_grid = ncu * 8;
+    r->slot_elems = slot_elems;
+    r->ctr_words = persist_ctr_words(0) + 3 * B;
     bool ok = hipMalloc(&r->d_items, sorted.size() * sizeof(RagItem)) == hipSuccess &&
               hipMalloc(&r->d_ids, ids.size() * sizeof(int)) == hipSuccess &&
               hipMalloc(&r->d_offs, offs.size() * sizeof(long long)) == hipSuccess &&
               hipMalloc(&r->d_tabs, tabs.size() * sizeof(RagTab)) == hipSuccess &&
-              hipMalloc(&r->d_counter, 64) == hipSuccess;
-    if (ok && max_scratch)
-        ok = hipMalloc(&r->scratch, (size_t)max_grid * max_scratch * cb) == hipSuccess;
+              hipMalloc((void **)&r->ctr, (size_t)r->ctr_words * sizeof(unsigned)) == hipSuccess &&
+              hipMemset(r->ctr, 0, (size_t)r->ctr_words * sizeof(unsigned)) == hipSuccess;
+    if (ok && B > 0) ok = hipMalloc(&r->scratch, (size_t)slots * slot_elems * cb) == hipSuccess;
     ok = ok && hipMemcpy(r->d_items, sorted.data(), sorted.size() * sizeof(RagItem), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(r->d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(r->d_offs, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice) == hipSuccess &&
@@ -766,5 +906,25 @@ int nfft_ragged_destroy(nfft_ragged *r)
 }
 
 long long nfft_ragged_total(const nfft_ragged *r) { return r ? r->total : 0; }
+
+int nfft_ragged_check(nfft_ragged *r, void *stream)
+{
+    clear_error();
+    if (!r) {
+        set_error("NULL ragged handle");
+        return NFFT_E_INVALID;
+    }
+    unsigned w[2] = {0, 0};
+    if (hipStreamSynchronize((hipStream_t)stream) != hipSuccess ||
+        hipMemcpy(w, r->ctr + 1, sizeof w, hipMemcpyDeviceToHost) != hipSuccess) {
+        set_error("hipStreamSynchronize / hipMemcpy failed");
+        return NFFT_E_HIP;
+    }
+    if (w[0] | w[1]) {
+        set_error("ragged batch: a tile dependency wait timed out");
+        return NFFT_E_HIP;
+    }
+    return NFFT_OK;
+}
 
 }  // extern "C"

@@ -24,9 +24,25 @@ def make(pkg, prec, sizes, flags=0):
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 def test_ragged_vs_pocketfft(pkg, dev, prec):
-    import torch
     sizes = ragged_sizes(300, seed=1)
     sizes += [1, 2, 3, 16, 4096, 4097, 8192, 65536, 65521, 2 * 3 * 5 * 7]
+    check_vs_pocketfft(pkg, dev, prec, sizes)
+
+
+@pytest.mark.parametrize("prec,mb,gpt,lag", [("f32", 2, 1, 8), ("f32", 4, 3, 1), ("f64", 4, 2, 8),
+                                             ("f64", 2, 64, 0)])
+def test_ragged_slot_reuse(pkg, dev, prec, mb, gpt, lag, monkeypatch):
+    """A scratch budget of a few slots: most phase-1 tiles wait for their
+    slot's previous user; tiles of one group up to whole phases."""
+    monkeypatch.setenv("NFFT_RAGGED_MB", str(mb))
+    monkeypatch.setenv("NFFT_RAGGED_GPT", str(gpt))
+    monkeypatch.setenv("NFFT_RAGGED_LAG", str(lag))
+    sizes = [65521, 8192, 40000, 65536, 5003, 16384, 32771, 9000, 30030, 65536, 12289, 4099, 16, 100]
+    check_vs_pocketfft(pkg, dev, prec, sizes)
+
+
+def check_vs_pocketfft(pkg, dev, prec, sizes):
+    import torch
     lib = pkg.core()
     h = make(pkg, prec, sizes)
     total = lib.nfft_ragged_total(h)
@@ -38,7 +54,7 @@ def test_ragged_vs_pocketfft(pkg, dev, prec):
     for inverse in (0, 1):
         torch.cuda.synchronize()
         pkg.check(lib.nfft_ragged_execute(h, inverse, xd.data_ptr(), yd.data_ptr(), None), "ragged_execute")
-        torch.cuda.synchronize()
+        pkg.check(lib.nfft_ragged_check(h, None), "ragged_check")
         y = yd.cpu().numpy().astype(np.float64)
         off = 0
         worst = 0.0
@@ -82,7 +98,7 @@ def test_ragged_graph_replays_match_eager(pkg, dev):
         junk = torch.randn(1 << 20, device=dev).sum()  # program's own work between replays
         torch.cuda.synchronize()
         pkg.check(lib.nfft_ragged_execute(h, 0, xd.data_ptr(), yd.data_ptr(), None), "ragged_execute")
-        torch.cuda.synchronize()
+        pkg.check(lib.nfft_ragged_check(h, None), "ragged_check")
         d = (yd - ref).abs().max().item()
         assert d <= 1e-4 * ref.abs().max().item(), (rep, d)
         del junk
