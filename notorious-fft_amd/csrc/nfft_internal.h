@@ -9,6 +9,7 @@
 
 #include "nfft_kernels.h"
 #include "nfft_kernels_persist.h"
+#include "nfft_kernels_smooth.h"
 #include "nfft_kernels_rt.h"
 
 namespace nfft {
@@ -103,6 +104,16 @@ bool persist_find(int prec, long long n, int variant, PersistCfg *cfg);
 hipError_t launch_four_persist(const PersistCfg &c, int dir, const PersistArgs &a, int grid, hipStream_t s);
 hipError_t launch_persist_reset(unsigned *ctr, long long words, hipStream_t s);
 
+// mixed-radix lines (nfft_k_smooth.hip); smooth_factor() builds the stage
+// schedule of a 7-smooth length (false otherwise)
+bool smooth_factor(long long n, SmoothPlan *sp);
+hipError_t launch_smooth_line(int prec, int dir, const SmoothArgs &a, hipStream_t s);
+// mixed-radix four-step: n = N1 x N2, both 7-smooth and <= maxlen
+bool smooth_split(long long n, int maxlen, SmoothPlan *sp1, SmoothPlan *sp2);
+hipError_t launch_smooth4(int prec, int dir, const Smooth4Args &a, hipStream_t s);
+// longest mixed-radix line one workgroup holds (two LDS buffers)
+inline int smooth_line_max(int prec) { return prec == F32 ? 8192 : 4096; }
+
 enum EwKind { EW_UNPACK = 0, EW_PACK = 1, EW_TRIG_PRE = 2, EW_TRIG_POST = 3, EW_N1 = 4 };
 hipError_t launch_ew(int prec, int kind, const EwArgs &a, long long nthreads_x, hipStream_t s);
 
@@ -139,7 +150,7 @@ const void *chirp_table(int prec, long long n, const double2 **chirp64);
 // ---------------------------------------------------------------------------
 // 1D complex engine over contiguous lines
 // ---------------------------------------------------------------------------
-enum C2CAlgo { ALG_LINE = 0, ALG_FOUR = 1, ALG_BLUE1 = 2, ALG_BLUE3 = 3, ALG_N1 = 4 };
+enum C2CAlgo { ALG_LINE = 0, ALG_FOUR = 1, ALG_BLUE1 = 2, ALG_BLUE3 = 3, ALG_N1 = 4, ALG_SMOOTH = 5, ALG_SMOOTH4 = 6 };
 
 struct C2C {
     int n = 0;
@@ -154,6 +165,10 @@ struct C2C {
     void *H = nullptr;   // owned: Bluestein filter spectrum (scaled 1/m)
     long long scratch_per_line = 0;  // elements of Cx per line (ALG_FOUR: n, ALG_BLUE3: m)
     int col_lpb = 0, row_lpb = 0;     // tile widths of the two passes (0: defaults)
+    SmoothPlan sp{};                  // ALG_SMOOTH: stage schedule; ALG_SMOOTH4: N1
+    SmoothPlan sp2{};                 // ALG_SMOOTH4: N2
+    int sgroup = 1;                   // ALG_SMOOTH: lines per workgroup; ALG_SMOOTH4: columns per tile
+    int sgroup2 = 1;                  // ALG_SMOOTH4: rows per tile
     // ALG_FOUR in one persistent launch (k_four_persist) when pctr is set
     PersistCfg pcfg{};
     unsigned *pctr = nullptr;         // owned: counter block for up to pcap transforms
@@ -165,7 +180,9 @@ struct C2C {
     long long pslots = 1;             // rotating slots at most (Infinity Cache budget)
 
     // Set up tables for length n; returns false with the error set.
-    bool init(int n, int prec);
+    // smooth_ok = false keeps Bluestein for non-power-of-two n (callers that
+    // need its chirp / filter tables)
+    bool init(int n, int prec, bool smooth_ok = true);
     void release();
     std::string describe() const;
     // dir: -1 forward, +1 inverse (unnormalised); scale applied to the output.

@@ -1,0 +1,135 @@
+// nfft_k_smooth.hip -- launchers of the mixed-radix kernels (both precisions).
+#include "nfft_internal.h"
+
+#include <cstdlib>
+
+namespace nfft {
+
+namespace {
+template <typename T, int DIR> void smooth_attr()
+{
+    static bool done = false;  // one attribute call per instantiation and process
+    if (!done) {
+        (void)hipFuncSetAttribute((const void *)k_smooth_line<T, DIR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipGetLastError();
+        done = true;
+    }
+}
+
+template <typename T> hipError_t smooth_line_impl(int dir, const SmoothArgs &a, hipStream_t s)
+{
+    const long long grid = (a.nlines + a.G - 1) / a.G;
+    const size_t lds = 2 * (size_t)a.G * a.sp.L * sizeof(Cx<T>);
+    if (grid < 1) return hipSuccess;
+    if (grid > 0x7fffffffLL || lds > 160 * 1024) return hipErrorInvalidValue;
+    if (dir < 0) {
+        smooth_attr<T, -1>();
+        hipLaunchKernelGGL((k_smooth_line<T, -1>), dim3((unsigned)grid), dim3(256), lds, s, a);
+    } else {
+        smooth_attr<T, 1>();
+        hipLaunchKernelGGL((k_smooth_line<T, 1>), dim3((unsigned)grid), dim3(256), lds, s, a);
+    }
+    return hipGetLastError();
+}
+template <typename T, int DIR> void smooth4_attr()
+{
+    static bool done = false;
+    if (!done) {
+        (void)hipFuncSetAttribute((const void *)k_smooth_col<T, DIR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_smooth_rowT<T, DIR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        (void)hipGetLastError();
+        done = true;
+    }
+}
+
+template <typename T> hipError_t smooth4_impl(int dir, const Smooth4Args &a, hipStream_t s)
+{
+    if (a.ntrans < 1) return hipSuccess;
+    const long long gy = a.ntrans < 65535 ? a.ntrans : 65535;
+    const long long gz = (a.ntrans + gy - 1) / gy;
+    const int N1 = a.sp1.L, N2 = a.sp2.L;
+    const dim3 g1((unsigned)((N2 + a.w1 - 1) / a.w1), (unsigned)gy, (unsigned)gz);
+    const dim3 g2((unsigned)((N1 + a.w2 - 1) / a.w2), (unsigned)gy, (unsigned)gz);
+    const size_t l1 = 2 * (size_t)a.w1 * N1 * sizeof(Cx<T>), l2 = 2 * (size_t)a.w2 * N2 * sizeof(Cx<T>);
+    if (gz > 65535 || l1 > 160 * 1024 || l2 > 160 * 1024) return hipErrorInvalidValue;
+    if (dir < 0) {
+        smooth4_attr<T, -1>();
+        hipLaunchKernelGGL((k_smooth_col<T, -1>), g1, dim3(256), l1, s, a);
+        hipLaunchKernelGGL((k_smooth_rowT<T, -1>), g2, dim3(256), l2, s, a);
+    } else {
+        smooth4_attr<T, 1>();
+        hipLaunchKernelGGL((k_smooth_col<T, 1>), g1, dim3(256), l1, s, a);
+        hipLaunchKernelGGL((k_smooth_rowT<T, 1>), g2, dim3(256), l2, s, a);
+    }
+    return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_smooth4(int prec, int dir, const Smooth4Args &a, hipStream_t s)
+{
+    const int slot = prof_on() ? prof_begin(s) : -1;
+    hipError_t e = prec == F32 ? smooth4_impl<float>(dir, a, s) : smooth4_impl<double>(dir, a, s);
+    if (slot >= 0)
+        prof_end(slot, s, "k_smooth_col+rowT", prec, a.sp1.L * a.sp2.L,
+                 4.0 * (double)a.ntrans * a.sp1.L * a.sp2.L * (prec == F32 ? 8.0 : 16.0));
+    return e;
+}
+
+bool smooth_split(long long n, int maxlen, SmoothPlan *sp1, SmoothPlan *sp2)
+{
+    // N1 = the divisor of n nearest sqrt(n) with both factors <= maxlen
+    long long best = 0;
+    for (long long d = 2; d * d <= n * 4 && d <= maxlen; d++) {
+        if (n % d) continue;
+        const long long e = n / d;
+        if (e > maxlen || e < 2) continue;
+        if (!best || std::llabs(d * d - n) < std::llabs(best * best - n)) best = d;
+    }
+    if (!best) return false;
+    return smooth_factor(best, sp1) && smooth_factor(n / best, sp2);
+}
+
+hipError_t launch_smooth_line(int prec, int dir, const SmoothArgs &a, hipStream_t s)
+{
+    const int slot = prof_on() ? prof_begin(s) : -1;
+    hipError_t e = prec == F32 ? smooth_line_impl<float>(dir, a, s) : smooth_line_impl<double>(dir, a, s);
+    if (slot >= 0)
+        prof_end(slot, s, "k_smooth_line", prec, a.sp.L, 2.0 * (double)a.nlines * a.sp.L * (prec == F32 ? 8.0 : 16.0));
+    return e;
+}
+
+bool smooth_factor(long long n, SmoothPlan *sp)
+{
+    if (n < 2 || n > (1LL << 30)) return false;
+    SmoothPlan p{};
+    p.L = (int)n;
+    long long m = n;
+    int ns = 0;
+    auto push = [&](int r) {
+        if (ns >= kSmoothMaxStages) return false;
+        p.radix[ns++] = (unsigned char)r;
+        m /= r;
+        return true;
+    };
+    while (m % 8 == 0)
+        if (!push(8)) return false;
+    while (m % 5 == 0)
+        if (!push(5)) return false;
+    while (m % 4 == 0)
+        if (!push(4)) return false;
+    while (m % 3 == 0)
+        if (!push(3)) return false;
+    while (m % 7 == 0)
+        if (!push(7)) return false;
+    while (m % 2 == 0)
+        if (!push(2)) return false;
+    if (m != 1) return false;
+    p.ns = ns;
+    *sp = p;
+    return true;
+}
+
+}  // namespace nfft

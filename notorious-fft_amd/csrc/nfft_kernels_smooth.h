@@ -1,0 +1,302 @@
+// nfft_kernels_smooth.h -- mixed-radix (2, 3, 4, 5, 7, 8) transforms for
+// lengths N = 2^a 3^b 5^c 7^d.
+//
+// The reference sends every length that is not a power of two through
+// Bluestein (src/05_algorithms.h:292-344: three power-of-two FFTs of
+// M >= 2N - 1 plus chirp passes).  Here a 7-smooth length is transformed
+// directly by a self-sorting Stockham sequence of radix-r stages held in LDS
+// (ping-pong buffers), one runtime schedule per length: stage s with radix r
+// and Ns = r_0 ... r_{s-1} maps butterfly j (k = j mod Ns) from inputs
+// x[j + t L/r] to outputs y[(j / Ns) Ns r + k + t Ns] after the twiddles
+// w_{Ns r}^{t k}, read from a table of e^{-2 pi i q / L} generated in FP64.
+// Each stage dispatches once (uniformly) on its radix to a compile-time
+// butterfly, so the per-thread values stay in registers.
+//
+// Used for whole lines (k_smooth_line, several lines per workgroup) and as
+// the column / row passes of a four-step for lengths above the LDS limit.
+#pragma once
+#include "nfft_kernels.h"
+#include "nfft_kernels_persist.h"  // coh_load / coh_store
+
+namespace nfft {
+
+constexpr int kSmoothMaxStages = 24;
+
+// Stage schedule of a 7-smooth length (radices in execution order).
+struct SmoothPlan {
+    int L;
+    int ns;
+    unsigned char radix[kSmoothMaxStages];
+};
+
+// Odd-radix DFTs from the symmetric form: with a_k = x_k + x_{P-k},
+// b_k = x_k - x_{P-k}, y_m = x_0 + sum_k cos(2 pi k m / P) a_k
+//                                -/+ i sum_k sin(2 pi k m / P) b_k (forward / inverse),
+// y_{P-m} with the opposite sign of the imaginary part.
+template <int P> struct OddTab;
+template <> struct OddTab<3> {
+    static constexpr double c[1][1] = {{-0.5}};
+    static constexpr double s[1][1] = {{0.86602540378443864676}};
+};
+template <> struct OddTab<5> {
+    static constexpr double c[2][2] = {{0.30901699437494742410, -0.80901699437494742410},
+                                       {-0.80901699437494742410, 0.30901699437494742410}};
+    static constexpr double s[2][2] = {{0.95105651629515357212, 0.58778525229247312917},
+                                       {0.58778525229247312917, -0.95105651629515357212}};
+};
+template <> struct OddTab<7> {
+    // c[m-1][k-1] = cos(2 pi k m / 7), s[m-1][k-1] = sin(2 pi k m / 7)
+    static constexpr double c[3][3] = {
+        {0.62348980185873353053, -0.22252093395631440429, -0.90096886790241912624},
+        {-0.22252093395631440429, -0.90096886790241912624, 0.62348980185873353053},
+        {-0.90096886790241912624, 0.62348980185873353053, -0.22252093395631440429}};
+    static constexpr double s[3][3] = {
+        {0.78183148246802980871, 0.97492791218182360702, 0.43388373911755812048},
+        {0.97492791218182360702, -0.43388373911755812048, -0.78183148246802980871},
+        {0.43388373911755812048, -0.78183148246802980871, 0.97492791218182360702}};
+};
+
+template <int P, int DIR, typename C> __device__ __forceinline__ void dft_odd(C *v)
+{
+    using R = decltype(C::x);
+    constexpr int H = P / 2;
+    C a[H], b[H];
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+        a[k] = cadd(v[k + 1], v[P - 1 - k]);
+        b[k] = csub(v[k + 1], v[P - 1 - k]);
+    }
+    C y0 = v[0];
+#pragma unroll
+    for (int k = 0; k < H; k++) y0 = cadd(y0, a[k]);
+#pragma unroll
+    for (int m = 0; m < H; m++) {
+        C t = v[0], u = cmk<C>((R)0, (R)0);
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            const R cc = (R)OddTab<P>::c[m][k], ss = (R)OddTab<P>::s[m][k];
+            t = cmk<C>(t.x + cc * a[k].x, t.y + cc * a[k].y);
+            u = cmk<C>(u.x + ss * b[k].x, u.y + ss * b[k].y);
+        }
+        // forward: y_m = t - i u, y_{P-m} = t + i u; inverse swaps them
+        const C iu = cmk<C>(-u.y, u.x);
+        if constexpr (DIR < 0) {
+            v[m + 1] = csub(t, iu);
+            v[P - 1 - m] = cadd(t, iu);
+        } else {
+            v[m + 1] = cadd(t, iu);
+            v[P - 1 - m] = csub(t, iu);
+        }
+    }
+    v[0] = y0;
+}
+
+template <int R, int DIR, typename C> __device__ __forceinline__ void dft_any(C *v)
+{
+    if constexpr (R == 3 || R == 5 || R == 7) dft_odd<R, DIR>(v);
+    else dftR<R, DIR>(v);
+}
+
+// One Stockham stage of radix R over G lines of length L (line g at
+// src/dst + g * L), all threads of the workgroup sharing the G*L/R butterflies.
+template <typename T, int R, int DIR>
+__device__ __forceinline__ void smooth_stage(const Cx<T> *src, Cx<T> *dst, int G, int L, int Ns, const Cx<T> *tw)
+{
+    using C = Cx<T>;
+    const int nb = L / R;
+    const int tws = L / (Ns * R);  // w_{Ns R}^{t k} = tw[t k tws], t k tws < L
+    for (int idx = threadIdx.x; idx < G * nb; idx += blockDim.x) {
+        const int g = idx / nb, j = idx - g * nb;
+        const int k = j % Ns;
+        const C *x = src + g * L + j;
+        C v[R];
+#pragma unroll
+        for (int t = 0; t < R; t++) v[t] = x[t * nb];
+        if (Ns > 1) {
+#pragma unroll
+            for (int t = 1; t < R; t++) v[t] = ctw<DIR>(v[t], tw[t * k * tws]);
+        }
+        dft_any<R, DIR>(v);
+        C *y = dst + g * L + (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int t = 0; t < R; t++) y[t * Ns] = v[t];
+    }
+}
+
+// All stages of G lines held in LDS buffer `a` (buffer `b` is the other half
+// of the ping-pong); returns the buffer holding the result.  All threads of
+// the workgroup must call it.
+template <typename T, int DIR>
+__device__ Cx<T> *smooth_lines(Cx<T> *a, Cx<T> *b, int G, const SmoothPlan &sp, const Cx<T> *tw)
+{
+    Cx<T> *src = a, *dst = b;
+    int Ns = 1;
+    for (int s = 0; s < sp.ns; s++) {
+        const int r = sp.radix[s];
+        switch (r) {
+        case 2: smooth_stage<T, 2, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 3: smooth_stage<T, 3, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 4: smooth_stage<T, 4, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 5: smooth_stage<T, 5, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 7: smooth_stage<T, 7, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        default: smooth_stage<T, 8, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        }
+        __syncthreads();
+        Cx<T> *t = src;
+        src = dst;
+        dst = t;
+        Ns *= r;
+    }
+    return src;
+}
+
+// ---------------------------------------------------------------------------
+// k_smooth_line: G lines of a 7-smooth length L per workgroup, contiguous
+// lines (line l at in + l * in_dist), out scaled by `scale`.
+// ---------------------------------------------------------------------------
+struct SmoothArgs {
+    const void *in;
+    void *out;
+    long long nlines;
+    long long in_dist, out_dist;
+    int G;            // lines per workgroup
+    SmoothPlan sp;
+    const void *tw;   // L entries e^{-2 pi i q / L}
+    double scale;
+};
+
+template <typename T, int DIR> __global__ void __launch_bounds__(256) k_smooth_line(SmoothArgs a)
+{
+    using C = Cx<T>;
+    extern __shared__ char smem[];
+    C *A = (C *)smem;
+    const int L = a.sp.L, G = a.G;
+    C *B = A + G * L;
+    const long long l0 = (long long)blockIdx.x * G;
+    const int g_here = (int)((a.nlines - l0) < G ? (a.nlines - l0) : G);
+    const C *src = (const C *)a.in;
+    for (int idx = threadIdx.x; idx < G * L; idx += blockDim.x) {
+        const int g = idx / L, p = idx - g * L;
+        A[idx] = g < g_here ? src[(l0 + g) * a.in_dist + p] : cmk<C>((T)0, (T)0);
+    }
+    __syncthreads();
+    const C *res = smooth_lines<T, DIR>(A, B, G, a.sp, (const C *)a.tw);
+    C *dst = (C *)a.out;
+    const T sc = (T)a.scale;
+    for (int idx = threadIdx.x; idx < G * L; idx += blockDim.x) {
+        const int g = idx / L, p = idx - g * L;
+        if (g < g_here) {
+            C v = res[idx];
+            if (sc != (T)1) v = cscale(v, sc);
+            dst[(l0 + g) * a.out_dist + p] = v;
+        }
+    }
+}
+
+}  // namespace nfft
+
+namespace nfft {
+
+// ---------------------------------------------------------------------------
+// Mixed-radix four-step tiles for N = N1 x N2 (both 7-smooth): element n of
+// a transform is x[n1 * N2 + n2].
+//   column tile: columns [c0, c0 + w) of the N1 x N2 view, FFT_N1 down each
+//     column, times w_N^{k1 c}, stored at scr[k1 * N2 + c];
+//   row tile: rows [r0, r0 + w) of scr (N2 contiguous points each), FFT_N2,
+//     stored transposed at out[k2 * N1 + r] (natural order of the result).
+// COH selects agent-scope (sc1) accesses for the intermediate (tiles that
+// other workgroups consume within the same launch).
+// ---------------------------------------------------------------------------
+template <typename T, int DIR, bool COH>
+__device__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, const SmoothPlan &sp1, int N2, int c0, int w,
+                                const Cx<T> *tw1, const Tw2 &tw2, Cx<T> *A, Cx<T> *B)
+{
+    using C = Cx<T>;
+    const int N1 = sp1.L;
+    for (int idx = threadIdx.x; idx < w * N1; idx += blockDim.x) {
+        const int p = idx / w, b = idx - p * w;
+        A[b * N1 + p] = in[(long long)p * N2 + c0 + b];
+    }
+    __syncthreads();
+    const C *res = smooth_lines<T, DIR>(A, B, w, sp1, tw1);
+    for (int idx = threadIdx.x; idx < w * N1; idx += blockDim.x) {
+        const int k1 = idx / w, b = idx - k1 * w;
+        const uint32_t e = (uint32_t)k1 * (uint32_t)(c0 + b);
+        const C v = ctw64<DIR>(res[b * N1 + k1], tw2_get(tw2, e));
+        C *d = scr + (long long)k1 * N2 + c0 + b;
+        if constexpr (COH) coh_store(d, v);
+        else *d = v;
+    }
+    __syncthreads();
+}
+
+template <typename T, int DIR, bool COH>
+__device__ void smooth_row_tile(const Cx<T> *scr, Cx<T> *out, const SmoothPlan &sp2, int N1, int r0, int w,
+                                const Cx<T> *tw2t, T scale, Cx<T> *A, Cx<T> *B)
+{
+    using C = Cx<T>;
+    const int N2 = sp2.L;
+    for (int idx = threadIdx.x; idx < w * N2; idx += blockDim.x) {
+        const C *s = scr + (long long)r0 * N2 + idx;
+        if constexpr (COH) A[idx] = coh_load(s);
+        else A[idx] = *s;
+    }
+    __syncthreads();
+    const C *res = smooth_lines<T, DIR>(A, B, w, sp2, tw2t);
+    for (int idx = threadIdx.x; idx < w * N2; idx += blockDim.x) {
+        const int k2 = idx / w, b = idx - k2 * w;
+        C v = res[b * N2 + k2];
+        if (scale != (T)1) v = cscale(v, scale);
+        out[(long long)k2 * N1 + r0 + b] = v;
+    }
+    __syncthreads();
+}
+
+// Standalone passes over a batch (grid.x: tiles of one transform, grid.y x
+// grid.z: transforms), the intermediate in a scratch of N per transform.
+struct Smooth4Args {
+    const void *in;
+    void *out;
+    void *scratch;
+    long long ntrans;
+    long long in_dist, out_dist;
+    SmoothPlan sp1, sp2;      // N1, N2
+    int w1, w2;               // columns / rows per tile
+    const void *tw1, *tw2t;   // e^{-2 pi i q / N1}, e^{-2 pi i q / N2}
+    Tw2 tw2;                  // e^{-2 pi i e / N}
+    double scale;
+};
+
+template <typename T, int DIR> __global__ void __launch_bounds__(256) k_smooth_col(Smooth4Args a)
+{
+    using C = Cx<T>;
+    extern __shared__ char smem[];
+    C *A = (C *)smem;
+    C *B = A + a.w1 * a.sp1.L;
+    const long long t = (long long)blockIdx.y + (long long)gridDim.y * blockIdx.z;
+    if (t >= a.ntrans) return;  // uniform per workgroup
+    const int N2 = a.sp2.L;
+    const long long N = (long long)a.sp1.L * N2;
+    const int c0 = blockIdx.x * a.w1;
+    const int w = (N2 - c0) < a.w1 ? (N2 - c0) : a.w1;
+    smooth_col_tile<T, DIR, false>((const C *)a.in + t * a.in_dist, (C *)a.scratch + t * N, a.sp1, N2, c0, w,
+                                   (const C *)a.tw1, a.tw2, A, B);
+}
+
+template <typename T, int DIR> __global__ void __launch_bounds__(256) k_smooth_rowT(Smooth4Args a)
+{
+    using C = Cx<T>;
+    extern __shared__ char smem[];
+    C *A = (C *)smem;
+    C *B = A + a.w2 * a.sp2.L;
+    const long long t = (long long)blockIdx.y + (long long)gridDim.y * blockIdx.z;
+    if (t >= a.ntrans) return;
+    const int N1 = a.sp1.L;
+    const long long N = (long long)N1 * a.sp2.L;
+    const int r0 = blockIdx.x * a.w2;
+    const int w = (N1 - r0) < a.w2 ? (N1 - r0) : a.w2;
+    smooth_row_tile<T, DIR, false>((const C *)a.scratch + t * N, (C *)a.out + t * a.out_dist, a.sp2, N1, r0, w,
+                                   (const C *)a.tw2t, (T)a.scale, A, B);
+}
+
+}  // namespace nfft

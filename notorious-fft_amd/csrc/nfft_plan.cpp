@@ -265,7 +265,7 @@ static bool persist_setup(C2C &c)
     return true;
 }
 
-bool C2C::init(int n_, int prec_)
+bool C2C::init(int n_, int prec_, bool smooth_ok)
 {
     n = n_;
     prec = prec_;
@@ -304,6 +304,27 @@ bool C2C::init(int n_, int prec_)
         }
         tw_a = stage_table(prec, n1);
         tw_b = stage_table(prec, n2);
+        return tw_a && tw_b && tw2_table(n, &tw2);
+    }
+    // 7-smooth lengths: mixed-radix lines (the reference uses Bluestein for
+    // every non-power-of-two length; results agree to rounding)
+    if (smooth_ok && env_int("NFFT_SMOOTH", 1) && n <= smooth_line_max(prec) && smooth_factor(n, &sp)) {
+        algo = ALG_SMOOTH;
+        L = n;
+        sgroup = std::max(1, (prec == F32 ? 4096 : 2048) / n);
+        tw_a = phase_table(prec, n, 1, 0, n);
+        return tw_a != nullptr;
+    }
+    if (smooth_ok && env_int("NFFT_SMOOTH", 1) && smooth_split(n, smooth_line_max(prec), &sp, &sp2)) {
+        algo = ALG_SMOOTH4;
+        n1 = sp.L;
+        n2 = sp2.L;
+        const int budget = prec == F32 ? 4096 : 2048;  // points per workgroup tile (64 KiB of LDS, two buffers)
+        sgroup = std::max(1, std::min(n2, budget / n1));
+        sgroup2 = std::max(1, std::min(n1, budget / n2));
+        tw_a = phase_table(prec, n1, 1, 0, n1);
+        tw_b = phase_table(prec, n2, 1, 0, n2);
+        scratch_per_line = n;
         return tw_a && tw_b && tw2_table(n, &tw2);
     }
     // Bluestein
@@ -385,6 +406,14 @@ std::string C2C::describe() const
             snprintf(b, sizeof b, "four-step %dx%d", n1, n2);
         break;
     case ALG_BLUE1: snprintf(b, sizeof b, "bluestein N=%d M=%d single-pass", n, m); break;
+    case ALG_SMOOTH: {
+        int o = snprintf(b, sizeof b, "mixed-radix %d (", n);
+        for (int i = 0; i < sp.ns && o < (int)sizeof b - 8; i++)
+            o += snprintf(b + o, sizeof b - o, i ? "x%d" : "%d", sp.radix[i]);
+        snprintf(b + o, sizeof b - o, ")");
+        break;
+    }
+    case ALG_SMOOTH4: snprintf(b, sizeof b, "mixed-radix four-step %dx%d", n1, n2); break;
     case ALG_BLUE3: snprintf(b, sizeof b, "bluestein N=%d M=%d (%dx%d)", n, m, m1, m2); break;
     default: snprintf(b, sizeof b, "n=%d", n);
     }
@@ -404,6 +433,33 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
     case ALG_BLUE1: {
         BlueArgs a{in, out, nlines, in_dist, out_dist, n, tw_a, chirp, H, dir > 0 ? 1 : 0, scale};
         return launch_blue_line(prec, m, a, s);
+    }
+    case ALG_SMOOTH: {
+        SmoothArgs a{in, out, nlines, in_dist, out_dist, sgroup, sp, tw_a, scale};
+        return launch_smooth_line(prec, dir, a, s);
+    }
+    case ALG_SMOOTH4: {
+        if (scratch_lines < 1 || !scratch) return hipErrorInvalidValue;
+        for (long long l0 = 0; l0 < nlines; l0 += scratch_lines) {
+            Smooth4Args a{};
+            a.in = (const char *)in + (size_t)l0 * in_dist * cb;
+            a.out = (char *)out + (size_t)l0 * out_dist * cb;
+            a.scratch = scratch;
+            a.ntrans = std::min(scratch_lines, nlines - l0);
+            a.in_dist = in_dist;
+            a.out_dist = out_dist;
+            a.sp1 = sp;
+            a.sp2 = sp2;
+            a.w1 = sgroup;
+            a.w2 = sgroup2;
+            a.tw1 = tw_a;
+            a.tw2t = tw_b;
+            a.tw2 = tw2;
+            a.scale = scale;
+            hipError_t e = launch_smooth4(prec, dir, a, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
     }
     case ALG_FOUR: {
         if (scratch_lines < 1 || !scratch) return hipErrorInvalidValue;

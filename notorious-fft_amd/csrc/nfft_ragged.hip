@@ -605,7 +605,7 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
             // for M <= line limit (natural) or M1 x M2 (permuted); the ragged
             // kernel needs natural order for M <= 4096 and the [k1][k2] order
             // for its own M1 x M2 split, so H is rebuilt here when they differ.
-            if (!e.init(n, prec)) { ragged_free(r); return NFFT_E_HIP; }
+            if (!e.init(n, prec, false)) { ragged_free(r); return NFFT_E_HIP; }
             t.chirp = e.chirp;
             if (m <= 4096) {
                 t.tw_a = stage_table(prec, (int)m);

@@ -237,3 +237,45 @@ def test_persist_grab(pkg, dev, grab, monkeypatch):
         for op in ("dft", "invdft"):
             got = run_plan(pkg, dev, op, [n], x, batch, prec)
             assert ol.rel_l2(got, ol.run_oracle(op, [n], x, batch)) < tol_for(prec, n), (prec, op)
+
+
+SMOOTH = [6, 9, 12, 15, 25, 27, 45, 49, 96, 100, 210, 243, 343, 625, 1000, 1715, 2187, 3000, 4725, 6561, 8000]
+
+
+@pytest.mark.parametrize("prec,n", [("f32", n) for n in SMOOTH] + [("f64", n) for n in SMOOTH if n <= 4096])
+def test_mixed_radix(pkg, dev, prec, n):
+    """7-smooth lengths run as mixed-radix lines (several lines per
+    workgroup, a ragged last workgroup): vs pocketfft, and the plan says so."""
+    batch = 7
+    x = rounded(ol.uniform(11, 2 * n * batch), prec)
+    hi = il(np.fft.fft(cx(x).reshape(batch, n), axis=1))
+    got = run_plan(pkg, dev, "dft", [n], x, batch, prec)
+    err = ol.rel_l2(got, hi)
+    print(f"{prec} mixed-radix N={n}: rel-L2 vs pocketfft {err:.3e}")
+    assert err < tol_for(prec, n)
+    goti = run_plan(pkg, dev, "invdft", [n], x, batch, prec)
+    assert ol.rel_l2(goti, il(np.fft.ifft(cx(x).reshape(batch, n), axis=1))) < tol_for(prec, n)
+    p = pkg.Plan("dft", prec, [n], batch)
+    assert p.describe().startswith("mixed-radix"), p.describe()
+    p.close()
+
+
+SMOOTH4 = [10000, 15625, 40000, 65610, 196608, 1000000, 531441]
+
+
+@pytest.mark.parametrize("prec,n", [("f32", n) for n in SMOOTH4] + [("f64", n) for n in SMOOTH4[:5]])
+def test_mixed_radix_four_step(pkg, dev, prec, n):
+    """7-smooth lengths above one workgroup: mixed-radix column pass with the
+    inter-pass twiddle, then row pass with a transposed store."""
+    batch = 3 if n <= 200000 else 1
+    x = rounded(ol.uniform(12, 2 * n * batch), prec)
+    hi = il(np.fft.fft(cx(x).reshape(batch, n), axis=1))
+    got = run_plan(pkg, dev, "dft", [n], x, batch, prec)
+    err = ol.rel_l2(got, hi)
+    print(f"{prec} mixed-radix four-step N={n}: rel-L2 vs pocketfft {err:.3e}")
+    assert err < tol_for(prec, n)
+    goti = run_plan(pkg, dev, "invdft", [n], x, batch, prec, inplace=True)
+    assert ol.rel_l2(goti, il(np.fft.ifft(cx(x).reshape(batch, n), axis=1))) < tol_for(prec, n)
+    p = pkg.Plan("dft", prec, [n], batch)
+    assert p.describe().startswith("mixed-radix four-step"), p.describe()
+    p.close()
