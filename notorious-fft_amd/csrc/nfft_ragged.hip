@@ -40,12 +40,15 @@
 namespace nfft {
 
 constexpr int RG_THREADS = 256;
-enum RagType { RG_LINE = 0, RG_BLUE1 = 1, RG_FOUR = 2, RG_BLUE3 = 3 };
+enum RagType { RG_LINE = 0, RG_BLUE1 = 1, RG_FOUR = 2, RG_BLUE3 = 3, RG_SMOOTH = 4, RG_SMOOTH4 = 5 };
+// longest mixed-radix line a ragged workgroup holds (two LDS buffers within
+// the kernel's dynamic LDS) and the points of one mixed-radix tile
+constexpr int RG_SMOOTH_MAX = 2048;
 
-// A work item: a group of short transforms (LINE, BLUE1), or one tile of a
-// long transform (FOUR: phase 1 column tiles, phase 2 row tiles; BLUE3:
-// phases 1-3 = column, row, column tiles), covering column/row groups
-// [g0, g1) of that phase.
+// A work item: a group of short transforms (LINE, BLUE1, SMOOTH), or one
+// tile of a long transform (FOUR, SMOOTH4: phase 1 column tiles, phase 2 row
+// tiles; BLUE3: phases 1-3 = column, row, column tiles), covering column/row
+// groups [g0, g1) of that phase.
 struct RagItem {
     int type;
     int L, L2;     // LINE: n; BLUE1: M; FOUR: n1 x n2; BLUE3: m1 x m2
@@ -65,6 +68,8 @@ struct RagTab {
     const void *tw_a, *tw_b;
     const void *chirp, *H;
     Tw2 tw2;
+    SmoothPlan sp1, sp2;   // mixed radix: line (sp1) or four-step N1 (sp1) x N2 (sp2)
+    int w1, w2;            // mixed-radix four-step: columns / rows per group
 };
 
 struct RagArgs {
@@ -326,6 +331,55 @@ __device__ void rg_blue3(const RagArgs &a, const RagItem &it, Cx<T> *lds)
         rg_cols<T, M1, M2, 1, 2>(scr, (C *)a.out + base, tb, it.n, cj, sc, lds, it.g0, it.g1);
 }
 
+// a group of short mixed-radix transforms (same length), LDS-resident
+template <typename T, int DIR>
+__device__ void rg_smooth_group(const RagArgs &a, const RagItem &it, Cx<T> *lds, long long *bases)
+{
+    using C = Cx<T>;
+    const RagTab &tb = a.tabs[it.tab];
+    const int L = tb.sp1.L, G = it.count;
+    if (threadIdx.x < G) bases[threadIdx.x] = a.offs[a.ids[it.first + threadIdx.x]];
+    __syncthreads();
+    C *A = lds, *B = lds + G * L;
+    for (int idx = threadIdx.x; idx < G * L; idx += RG_THREADS) {
+        const int g = idx / L, p = idx - g * L;
+        A[idx] = ((const C *)a.in)[bases[g] + p];
+    }
+    __syncthreads();
+    const C *res = smooth_lines<T, DIR>(A, B, G, tb.sp1, (const C *)tb.tw_a);
+    const T sc = DIR > 0 ? (T)1 / (T)L : (T)1;  // inverse: 1/N for non-powers of two
+    for (int idx = threadIdx.x; idx < G * L; idx += RG_THREADS) {
+        const int g = idx / L, p = idx - g * L;
+        ((C *)a.out)[bases[g] + p] = cscale(res[idx], sc);
+    }
+    __syncthreads();
+}
+
+// one tile of a long mixed-radix transform (four-step N1 x N2)
+template <typename T, int DIR>
+__device__ void rg_smooth4(const RagArgs &a, const RagItem &it, Cx<T> *lds)
+{
+    using C = Cx<T>;
+    const RagTab &tb = a.tabs[it.tab];
+    const long long base = a.offs[a.ids[it.first]];
+    C *scr = (C *)a.scratch + (long long)it.slot * a.slot_elems;
+    const int N1 = tb.sp1.L, N2 = tb.sp2.L;
+    if (it.phase == 1) {
+        for (int g = it.g0; g < it.g1; g++) {
+            const int c0 = g * tb.w1, w = (N2 - c0) < tb.w1 ? (N2 - c0) : tb.w1;
+            smooth_col_tile<T, DIR, true>((const C *)a.in + base, scr, tb.sp1, N2, c0, w, (const C *)tb.tw_a, tb.tw2,
+                                          lds, lds + w * N1);
+        }
+    } else {
+        const T sc = DIR > 0 ? (T)1 / (T)((long long)N1 * N2) : (T)1;
+        for (int g = it.g0; g < it.g1; g++) {
+            const int r0 = g * tb.w2, w = (N1 - r0) < tb.w2 ? (N1 - r0) : tb.w2;
+            smooth_row_tile<T, DIR, true>(scr, (C *)a.out + base, tb.sp2, N1, r0, w, (const C *)tb.tw_b, sc, lds,
+                                          lds + w * N2);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // persistent kernel
 // ---------------------------------------------------------------------------
@@ -392,6 +446,8 @@ __global__ void __launch_bounds__(RG_THREADS) k_ragged(RagArgs a)
             case 256 * 100000 + 256: rg_four<T, 256, 256, DIR>(a, it, lds); break;
             }
             break;
+        case RG_SMOOTH: rg_smooth_group<T, DIR>(a, it, lds, bases); break;
+        case RG_SMOOTH4: rg_smooth4<T, DIR>(a, it, lds); break;
         case RG_BLUE3:
             switch (it.L * 100000 + it.L2) {
             case 64 * 100000 + 128: rg_blue3<T, 64, 128, DIR>(a, it, lds); break;
@@ -405,7 +461,7 @@ __global__ void __launch_bounds__(RG_THREADS) k_ragged(RagArgs a)
         // tiles that wrote the intermediate: each wave's stores are complete
         // (acknowledged at agent scope) before the barrier that precedes
         // lane 0's counter increment
-        if ((it.type == RG_FOUR && it.phase == 1) || (it.type == RG_BLUE3 && it.phase < 3))
+        if (((it.type == RG_FOUR || it.type == RG_SMOOTH4) && it.phase == 1) || (it.type == RG_BLUE3 && it.phase < 3))
             __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
     }
@@ -569,6 +625,7 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
     const size_t cb = cx_bytes(prec);
     std::vector<long long> offs(count);
     std::map<int, int> size_tab;  // size -> table index
+    std::map<int, int> size_kind; // size -> item type
     std::vector<RagTab> tabs;
     for (int i = 0; i < count; i++) {
         const int n = sizes[i];
@@ -591,10 +648,29 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
             return NFFT_E_UNSUPPORTED;
         }
         RagTab t{};
-        if (p2 && n <= 4096) {
+        SmoothPlan sp1{}, sp2{};
+        const bool smooth_line = !p2 && n <= RG_SMOOTH_MAX && smooth_factor(n, &sp1);
+        const bool smooth_four = !p2 && !smooth_line && smooth_split(n, RG_SMOOTH_MAX, &sp1, &sp2);
+        if (smooth_line) {
+            size_kind[n] = RG_SMOOTH;
+            t.sp1 = sp1;
+            t.tw_a = phase_table(prec, n, 1, 0, n);
+            if (!t.tw_a) { ragged_free(r); return NFFT_E_HIP; }
+        } else if (smooth_four) {
+            size_kind[n] = RG_SMOOTH4;
+            t.sp1 = sp1;
+            t.sp2 = sp2;
+            t.w1 = std::max(1, std::min(sp2.L, RG_SMOOTH_MAX / sp1.L));
+            t.w2 = std::max(1, std::min(sp1.L, RG_SMOOTH_MAX / sp2.L));
+            t.tw_a = phase_table(prec, sp1.L, 1, 0, sp1.L);
+            t.tw_b = phase_table(prec, sp2.L, 1, 0, sp2.L);
+            if (!t.tw_a || !t.tw_b || !tw2_table(n, &t.tw2)) { ragged_free(r); return NFFT_E_HIP; }
+        } else if (p2 && n <= 4096) {
+            size_kind[n] = RG_LINE;
             t.tw_a = stage_table(prec, n);
             if (!t.tw_a) { ragged_free(r); return NFFT_E_HIP; }
         } else if (p2) {
+            size_kind[n] = RG_FOUR;
             int n1, n2;
             four_split(n, &n1, &n2);
             t.tw_a = stage_table(prec, n1);
@@ -607,6 +683,7 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
             // for its own M1 x M2 split, so H is rebuilt here when they differ.
             if (!e.init(n, prec, false)) { ragged_free(r); return NFFT_E_HIP; }
             t.chirp = e.chirp;
+            size_kind[n] = m <= 4096 ? RG_BLUE1 : RG_BLUE3;
             if (m <= 4096) {
                 t.tw_a = stage_table(prec, (int)m);
                 if (e.algo != ALG_BLUE1) { set_error("unexpected Bluestein layout"); ragged_free(r); return NFFT_E_HIP; }
@@ -653,7 +730,8 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
     // (FOUR, BLUE3), each long transform split into phases of column/row
     // tiles that many workgroups share
     struct Long {
-        int id, type, L, L2, n, tab, phases, groups, tiles;
+        int id, type, L, L2, n, tab, phases;
+        int groups[3], tiles[3];  // per phase
         long long elems;
         double cost;
     };
@@ -661,41 +739,47 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
     std::map<std::pair<int, int>, std::vector<int>> groups;  // (type, size) -> ids
     for (int i = 0; i < count; i++) {
         const int n = sizes[i];
+        const int kind = size_kind[n];
         long long m = 1;
         while (m < 2LL * n - 1) m <<= 1;
-        if (is_pow2(n) && n > 4096) {
-            Long l{};
-            l.id = i;
-            l.type = RG_FOUR;
+        Long l{};
+        l.id = i;
+        l.type = kind;
+        l.n = n;
+        l.tab = size_tab[n];
+        if (kind == RG_FOUR) {
             four_split(n, &l.L, &l.L2);
-            l.n = n;
-            l.tab = size_tab[n];
             l.phases = 2;
-            l.groups = n / (RG_THREADS * 16);
+            l.groups[0] = l.groups[1] = n / (RG_THREADS * 16);
             l.elems = n;
             l.cost = 4.0 * n;
             longs.push_back(l);
-        } else if (!is_pow2(n) && m > 4096) {
-            Long l{};
-            l.id = i;
-            l.type = RG_BLUE3;
+        } else if (kind == RG_BLUE3) {
             four_split((int)m, &l.L, &l.L2);
-            l.n = n;
-            l.tab = size_tab[n];
             l.phases = 3;
-            l.groups = (int)(m / (RG_THREADS * 16));
+            l.groups[0] = l.groups[1] = l.groups[2] = (int)(m / (RG_THREADS * 16));
             l.elems = m;
             l.cost = 6.0 * m;
             longs.push_back(l);
+        } else if (kind == RG_SMOOTH4) {
+            const RagTab &t = tabs[size_tab[n]];
+            l.L = t.sp1.L;
+            l.L2 = t.sp2.L;
+            l.phases = 2;
+            l.groups[0] = (t.sp2.L + t.w1 - 1) / t.w1;
+            l.groups[1] = (t.sp1.L + t.w2 - 1) / t.w2;
+            l.elems = n;
+            l.cost = 4.0 * n;
+            longs.push_back(l);
         } else {
-            groups[{is_pow2(n) ? RG_LINE : RG_BLUE1, n}].push_back(i);
+            groups[{kind, n}].push_back(i);
         }
     }
     std::stable_sort(longs.begin(), longs.end(), [](const Long &a, const Long &b) { return a.cost > b.cost; });
     const int gpt = std::max(1, env_int_rg("NFFT_RAGGED_GPT", 4));  // column/row groups per tile (measured best)
     long long slot_elems = 0;
     for (Long &l : longs) {
-        l.tiles = (l.groups + gpt - 1) / gpt;
+        for (int ph = 0; ph < l.phases; ph++) l.tiles[ph] = (l.groups[ph] + gpt - 1) / gpt;
         slot_elems = std::max(slot_elems, l.elems);
     }
     const long long B = (long long)longs.size();
@@ -718,7 +802,7 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
     }
     auto emit = [&](long long b, int phase) {
         const Long &l = longs[b];
-        for (int t = 0; t < l.tiles; t++) {
+        for (int t = 0; t < l.tiles[phase - 1]; t++) {
             RagItem it{};
             it.type = l.type;
             it.L = l.L;
@@ -729,16 +813,16 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
             it.tab = l.tab;
             it.phase = phase;
             it.g0 = t * gpt;
-            it.g1 = std::min(l.groups, (t + 1) * gpt);
+            it.g1 = std::min(l.groups[phase - 1], (t + 1) * gpt);
             it.slot = (int)(b % slots);
             it.done = (int)(3 * b + phase - 1);
             if (phase > 1) {
                 it.wait = (int)(3 * b + phase - 2);
-                it.need = l.tiles;
+                it.need = l.tiles[phase - 2];
             } else if (b >= slots) {
                 const Long &prev = longs[b - slots];
                 it.wait = (int)(3 * (b - slots) + prev.phases - 1);
-                it.need = prev.tiles;
+                it.need = prev.tiles[prev.phases - 1];
             } else {
                 it.wait = -1;
             }
@@ -757,10 +841,11 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
         const std::vector<int> &v = kv.second;
         long long m = 1;
         while (m < 2LL * n - 1) m <<= 1;
-        const int L = type == RG_LINE ? n : (int)m;
+        const int L = type == RG_BLUE1 ? (int)m : n;
         int per = RG_THREADS / std::max(1, L / 16);
         if (L < 16) per = RG_THREADS;
-        const double c1 = type == RG_LINE ? 2.0 * n : 2.0 * n + 0.5 * m;
+        if (type == RG_SMOOTH) per = std::max(1, std::min(RG_THREADS, RG_SMOOTH_MAX / n));
+        const double c1 = type == RG_BLUE1 ? 2.0 * n + 0.5 * m : 2.0 * n;
         for (size_t k = 0; k < v.size(); k += per) {
             RagItem it{};
             it.type = type;
