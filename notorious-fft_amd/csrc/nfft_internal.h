@@ -111,8 +111,12 @@ hipError_t launch_smooth_line(int prec, int dir, const SmoothArgs &a, hipStream_
 // mixed-radix four-step: n = N1 x N2, both 7-smooth and <= maxlen
 bool smooth_split(long long n, int maxlen, SmoothPlan *sp1, SmoothPlan *sp2);
 hipError_t launch_smooth4(int prec, int dir, const Smooth4Args &a, hipStream_t s);
-// longest mixed-radix line one workgroup holds (two LDS buffers)
-inline int smooth_line_max(int prec) { return prec == F32 ? 8192 : 4096; }
+// longest mixed-radix line run by one workgroup (longer 7-smooth lengths
+// use the mixed-radix four-step), and the points per workgroup of the line
+// and tile kernels (two LDS buffers each: 32 KiB, so that five workgroups
+// share a CU)
+int smooth_line_max(int prec);
+inline int smooth_tile_points(int prec) { return prec == F32 ? 2048 : 1024; }
 
 enum EwKind { EW_UNPACK = 0, EW_PACK = 1, EW_TRIG_PRE = 2, EW_TRIG_POST = 3, EW_N1 = 4 };
 hipError_t launch_ew(int prec, int kind, const EwArgs &a, long long nthreads_x, hipStream_t s);

@@ -1,6 +1,7 @@
 // nfft_k_smooth.hip -- launchers of the mixed-radix kernels (both precisions).
 #include "nfft_internal.h"
 
+#include <algorithm>
 #include <cstdlib>
 
 namespace nfft {
@@ -99,6 +100,13 @@ hipError_t launch_smooth_line(int prec, int dir, const SmoothArgs &a, hipStream_
     if (slot >= 0)
         prof_end(slot, s, "k_smooth_line", prec, a.sp.L, 2.0 * (double)a.nlines * a.sp.L * (prec == F32 ? 8.0 : 16.0));
     return e;
+}
+
+int smooth_line_max(int prec)
+{
+    const char *v = getenv("NFFT_SMOOTH_LINE_MAX");
+    if (v && *v && atoi(v) >= 2) return std::min(atoi(v), prec == F32 ? 8192 : 4096);
+    return prec == F32 ? 2048 : 1024;
 }
 
 bool smooth_factor(long long n, SmoothPlan *sp)

@@ -91,6 +91,16 @@ template <int P, int DIR, typename C> __device__ __forceinline__ void dft_odd(C 
     v[0] = y0;
 }
 
+// n / d for 0 <= n < 2^24 and d >= 1, via the FP32 reciprocal and one
+// correction step (the estimate is within one of the quotient there).
+__device__ __forceinline__ int fdiv(int n, int d, float inv)
+{
+    int q = (int)((float)n * inv);
+    const int r = n - q * d;
+    q += (r >= d) - (r < 0);
+    return q;
+}
+
 template <int R, int DIR, typename C> __device__ __forceinline__ void dft_any(C *v)
 {
     if constexpr (R == 3 || R == 5 || R == 7) dft_odd<R, DIR>(v);
@@ -105,9 +115,10 @@ __device__ __forceinline__ void smooth_stage(const Cx<T> *src, Cx<T> *dst, int G
     using C = Cx<T>;
     const int nb = L / R;
     const int tws = L / (Ns * R);  // w_{Ns R}^{t k} = tw[t k tws], t k tws < L
+    const float inb = 1.0f / (float)nb, ins = 1.0f / (float)Ns;
     for (int idx = threadIdx.x; idx < G * nb; idx += blockDim.x) {
-        const int g = idx / nb, j = idx - g * nb;
-        const int k = j % Ns;
+        const int g = fdiv(idx, nb, inb), j = idx - g * nb;
+        const int jq = fdiv(j, Ns, ins), k = j - jq * Ns;
         const C *x = src + g * L + j;
         C v[R];
 #pragma unroll
@@ -117,7 +128,7 @@ __device__ __forceinline__ void smooth_stage(const Cx<T> *src, Cx<T> *dst, int G
             for (int t = 1; t < R; t++) v[t] = ctw<DIR>(v[t], tw[t * k * tws]);
         }
         dft_any<R, DIR>(v);
-        C *y = dst + g * L + (j / Ns) * Ns * R + k;
+        C *y = dst + g * L + jq * Ns * R + k;
 #pragma unroll
         for (int t = 0; t < R; t++) y[t * Ns] = v[t];
     }
@@ -175,8 +186,9 @@ template <typename T, int DIR> __global__ void __launch_bounds__(256) k_smooth_l
     const long long l0 = (long long)blockIdx.x * G;
     const int g_here = (int)((a.nlines - l0) < G ? (a.nlines - l0) : G);
     const C *src = (const C *)a.in;
+    const float iL = 1.0f / (float)L;
     for (int idx = threadIdx.x; idx < G * L; idx += blockDim.x) {
-        const int g = idx / L, p = idx - g * L;
+        const int g = fdiv(idx, L, iL), p = idx - g * L;
         A[idx] = g < g_here ? src[(l0 + g) * a.in_dist + p] : cmk<C>((T)0, (T)0);
     }
     __syncthreads();
@@ -184,7 +196,7 @@ template <typename T, int DIR> __global__ void __launch_bounds__(256) k_smooth_l
     C *dst = (C *)a.out;
     const T sc = (T)a.scale;
     for (int idx = threadIdx.x; idx < G * L; idx += blockDim.x) {
-        const int g = idx / L, p = idx - g * L;
+        const int g = fdiv(idx, L, iL), p = idx - g * L;
         if (g < g_here) {
             C v = res[idx];
             if (sc != (T)1) v = cscale(v, sc);
@@ -213,14 +225,15 @@ __device__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, const SmoothPlan &s
 {
     using C = Cx<T>;
     const int N1 = sp1.L;
+    const float iw = 1.0f / (float)w;
     for (int idx = threadIdx.x; idx < w * N1; idx += blockDim.x) {
-        const int p = idx / w, b = idx - p * w;
+        const int p = fdiv(idx, w, iw), b = idx - p * w;
         A[b * N1 + p] = in[(long long)p * N2 + c0 + b];
     }
     __syncthreads();
     const C *res = smooth_lines<T, DIR>(A, B, w, sp1, tw1);
     for (int idx = threadIdx.x; idx < w * N1; idx += blockDim.x) {
-        const int k1 = idx / w, b = idx - k1 * w;
+        const int k1 = fdiv(idx, w, iw), b = idx - k1 * w;
         const uint32_t e = (uint32_t)k1 * (uint32_t)(c0 + b);
         const C v = ctw64<DIR>(res[b * N1 + k1], tw2_get(tw2, e));
         C *d = scr + (long long)k1 * N2 + c0 + b;
@@ -243,8 +256,9 @@ __device__ void smooth_row_tile(const Cx<T> *scr, Cx<T> *out, const SmoothPlan &
     }
     __syncthreads();
     const C *res = smooth_lines<T, DIR>(A, B, w, sp2, tw2t);
+    const float iw = 1.0f / (float)w;
     for (int idx = threadIdx.x; idx < w * N2; idx += blockDim.x) {
-        const int k2 = idx / w, b = idx - k2 * w;
+        const int k2 = fdiv(idx, w, iw), b = idx - k2 * w;
         C v = res[b * N2 + k2];
         if (scale != (T)1) v = cscale(v, scale);
         out[(long long)k2 * N1 + r0 + b] = v;

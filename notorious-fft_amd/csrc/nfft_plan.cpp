@@ -311,15 +311,17 @@ bool C2C::init(int n_, int prec_, bool smooth_ok)
     if (smooth_ok && env_int("NFFT_SMOOTH", 1) && n <= smooth_line_max(prec) && smooth_factor(n, &sp)) {
         algo = ALG_SMOOTH;
         L = n;
-        sgroup = std::max(1, (prec == F32 ? 4096 : 2048) / n);
+        sgroup = std::max(1, smooth_tile_points(prec) / n);
         tw_a = phase_table(prec, n, 1, 0, n);
         return tw_a != nullptr;
     }
-    if (smooth_ok && env_int("NFFT_SMOOTH", 1) && smooth_split(n, smooth_line_max(prec), &sp, &sp2)) {
+    // (factors up to the one-workgroup line limit, else up to the LDS limit)
+    if (smooth_ok && env_int("NFFT_SMOOTH", 1) &&
+        (smooth_split(n, smooth_line_max(prec), &sp, &sp2) || smooth_split(n, prec == F32 ? 8192 : 4096, &sp, &sp2))) {
         algo = ALG_SMOOTH4;
         n1 = sp.L;
         n2 = sp2.L;
-        const int budget = prec == F32 ? 4096 : 2048;  // points per workgroup tile (64 KiB of LDS, two buffers)
+        const int budget = smooth_tile_points(prec);
         sgroup = std::max(1, std::min(n2, budget / n1));
         sgroup2 = std::max(1, std::min(n1, budget / n2));
         tw_a = phase_table(prec, n1, 1, 0, n1);
