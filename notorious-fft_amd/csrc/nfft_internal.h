@@ -184,9 +184,10 @@ struct C2C {
     long long pslots = 1;             // rotating slots at most (Infinity Cache budget)
 
     // Set up tables for length n; returns false with the error set.
-    // smooth_ok = false keeps Bluestein for non-power-of-two n (callers that
-    // need its chirp / filter tables)
-    bool init(int n, int prec, bool smooth_ok = true);
+    // smooth_mode for 7-smooth non-power-of-two n: 2 = choose (tuned /
+    // measured rule), 1 = mixed radix, 0 = Bluestein (callers that need its
+    // chirp / filter tables)
+    bool init(int n, int prec, int smooth_mode = 2);
     void release();
     std::string describe() const;
     // dir: -1 forward, +1 inverse (unnormalised); scale applied to the output.

@@ -227,6 +227,100 @@ static int env_int(const char *name, int dflt)
     return v && *v ? atoi(v) : dflt;
 }
 
+// ---------------------------------------------------------------------------
+// mixed radix vs Bluestein for 7-smooth lengths
+// ---------------------------------------------------------------------------
+// Tile width for `len` columns/rows at most `wmax` wide, balanced so that no
+// tile is much narrower than the others.
+static int balanced_width(int len, int wmax)
+{
+    wmax = std::max(1, std::min(len, wmax));
+    const int tiles = (len + wmax - 1) / wmax;
+    return (len + tiles - 1) / tiles;
+}
+
+// Measured on MI355X (2^28 FP32 / 2^27 FP64 elements): mixed-radix lines beat
+// Bluestein for every FP64 length and for FP32 up to 512 points; the
+// mixed-radix four-step wins for FP64 unless its tiles are a single column
+// or row wide, and for FP32 from 10^4 points with tiles >= 4 wide.
+static bool smooth_default(int prec, long long n, bool line, int wmin)
+{
+    if (line) return prec == F64 || n <= 512;
+    return prec == F64 ? wmin >= 2 : (n >= 10000 && wmin >= 4);
+}
+
+static std::string smooth_key(int prec, long long n)
+{
+    return "c2c-smooth p" + std::to_string(prec) + " n" + std::to_string(n);
+}
+
+// Choice recorded in the NFFT_TUNE_CACHE file (-1: none).
+static int tuned_choice(int prec, long long n)
+{
+    const char *path = getenv("NFFT_TUNE_CACHE");
+    if (!path || !*path) return -1;
+    FILE *f = fopen(path, "r");
+    if (!f) return -1;
+    const std::string key = smooth_key(prec, n);
+    char line[256];
+    int found = -1;
+    while (fgets(line, sizeof line, f)) {
+        if (strncmp(line, key.c_str(), key.size()) == 0 && line[key.size()] == ' ') found = atoi(line + key.size() + 1);
+    }
+    fclose(f);
+    return found;
+}
+
+// Project-written autotuner: times the mixed-radix plan and the Bluestein
+// plan of length n on a synthetic batch of about 2^22 points and records the
+// faster one (1: mixed radix, 0: Bluestein) in NFFT_TUNE_CACHE.
+static int tune_smooth(int n, int prec)
+{
+    C2C cand[2];
+    if (!cand[0].init(n, prec, 0) || !cand[1].init(n, prec, 1)) {
+        cand[0].release();
+        cand[1].release();
+        return -1;
+    }
+    const long long lines = std::max(1LL, (1LL << 22) / n);
+    const size_t cb = cx_bytes(prec);
+    hipStream_t st = setup_stream();
+    void *x = nullptr, *y = nullptr, *scr = nullptr;
+    const long long sp = std::max(cand[0].scratch_per_line, cand[1].scratch_per_line);
+    float ms[2] = {1e30f, 1e30f};
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = hipMalloc(&x, (size_t)lines * n * cb) == hipSuccess && hipMalloc(&y, (size_t)lines * n * cb) == hipSuccess &&
+              (sp == 0 || hipMalloc(&scr, (size_t)lines * sp * cb) == hipSuccess) && hipEventCreate(&e0) == hipSuccess &&
+              hipEventCreate(&e1) == hipSuccess &&
+              launch_fill_uniform(x, 2 * lines * n, 5, 0, prec, st) == hipSuccess;
+    for (int c = 0; c < 2 && ok; c++) {
+        for (int w = 0; w < 2 && ok; w++) ok = cand[c].run(x, y, lines, n, n, -1, 1.0, scr, lines, st) == hipSuccess;
+        ok = ok && hipEventRecord(e0, st) == hipSuccess;
+        for (int r = 0; r < 5 && ok; r++) ok = cand[c].run(x, y, lines, n, n, -1, 1.0, scr, lines, st) == hipSuccess;
+        ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&ms[c], e0, e1) == hipSuccess;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (void *p : {x, y, scr})
+        if (p) (void)hipFree(p);
+    cand[0].release();
+    cand[1].release();
+    if (!ok) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    const int pick = ms[1] < ms[0] ? 1 : 0;
+    const char *path = getenv("NFFT_TUNE_CACHE");
+    if (path && *path) {
+        if (FILE *f = fopen(path, "a")) {
+            fprintf(f, "%s %d\n", smooth_key(prec, n).c_str(), pick);
+            fclose(f);
+        }
+    }
+    return pick;
+}
+
 // Dependency waits of the persistent four-step give up after this long
 // (wall clock); a healthy wait lasts microseconds.
 static const double kPersistTimeoutSec = 2.0;
@@ -265,7 +359,7 @@ static bool persist_setup(C2C &c)
     return true;
 }
 
-bool C2C::init(int n_, int prec_, bool smooth_ok)
+bool C2C::init(int n_, int prec_, int smooth_mode)
 {
     n = n_;
     prec = prec_;
@@ -306,28 +400,45 @@ bool C2C::init(int n_, int prec_, bool smooth_ok)
         tw_b = stage_table(prec, n2);
         return tw_a && tw_b && tw2_table(n, &tw2);
     }
-    // 7-smooth lengths: mixed-radix lines (the reference uses Bluestein for
-    // every non-power-of-two length; results agree to rounding)
-    if (smooth_ok && env_int("NFFT_SMOOTH", 1) && n <= smooth_line_max(prec) && smooth_factor(n, &sp)) {
-        algo = ALG_SMOOTH;
-        L = n;
-        sgroup = std::max(1, smooth_tile_points(prec) / n);
-        tw_a = phase_table(prec, n, 1, 0, n);
-        return tw_a != nullptr;
-    }
-    // (factors up to the one-workgroup line limit, else up to the LDS limit)
-    if (smooth_ok && env_int("NFFT_SMOOTH", 1) &&
-        (smooth_split(n, smooth_line_max(prec), &sp, &sp2) || smooth_split(n, prec == F32 ? 8192 : 4096, &sp, &sp2))) {
-        algo = ALG_SMOOTH4;
-        n1 = sp.L;
-        n2 = sp2.L;
-        const int budget = smooth_tile_points(prec);
-        sgroup = std::max(1, std::min(n2, budget / n1));
-        sgroup2 = std::max(1, std::min(n1, budget / n2));
-        tw_a = phase_table(prec, n1, 1, 0, n1);
-        tw_b = phase_table(prec, n2, 1, 0, n2);
-        scratch_per_line = n;
-        return tw_a && tw_b && tw2_table(n, &tw2);
+    // 7-smooth lengths: mixed radix (the reference uses Bluestein for every
+    // non-power-of-two length; results agree to rounding) where it is the
+    // faster of the two: a tuned choice from NFFT_TUNE_CACHE, a measurement
+    // when NFFT_TUNE=1, else the measured default rule
+    const int smooth_env = env_int("NFFT_SMOOTH", 1);  // 0: never, 2: whenever possible
+    if (smooth_env == 2 && smooth_mode == 2) smooth_mode = 1;
+    if (smooth_mode != 0 && smooth_env != 0 && !is_pow2(n)) {
+        SmoothPlan p1{}, p2{};
+        const bool line = n <= smooth_line_max(prec) && smooth_factor(n, &p1);
+        const bool four = !line && (smooth_split(n, smooth_line_max(prec), &p1, &p2) ||
+                                    smooth_split(n, prec == F32 ? 8192 : 4096, &p1, &p2));
+        if (line || four) {
+            const int budget = smooth_tile_points(prec);
+            const int w1 = four ? balanced_width(p2.L, budget / p1.L) : 0;
+            const int w2 = four ? balanced_width(p1.L, budget / p2.L) : 0;
+            int use = smooth_mode == 1 ? 1 : tuned_choice(prec, n);
+            if (use < 0 && env_int("NFFT_TUNE", 0)) use = tune_smooth(n, prec);
+            if (use < 0) use = smooth_default(prec, n, line, std::min(w1, w2)) ? 1 : 0;
+            if (use == 1) {
+                sp = p1;
+                if (line) {
+                    algo = ALG_SMOOTH;
+                    L = n;
+                    sgroup = std::max(1, budget / n);
+                    tw_a = phase_table(prec, n, 1, 0, n);
+                    return tw_a != nullptr;
+                }
+                algo = ALG_SMOOTH4;
+                sp2 = p2;
+                n1 = p1.L;
+                n2 = p2.L;
+                sgroup = w1;
+                sgroup2 = w2;
+                tw_a = phase_table(prec, n1, 1, 0, n1);
+                tw_b = phase_table(prec, n2, 1, 0, n2);
+                scratch_per_line = n;
+                return tw_a && tw_b && tw2_table(n, &tw2);
+            }
+        }
     }
     // Bluestein
     m = 1;

@@ -660,8 +660,13 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
             size_kind[n] = RG_SMOOTH4;
             t.sp1 = sp1;
             t.sp2 = sp2;
-            t.w1 = std::max(1, std::min(sp2.L, RG_SMOOTH_MAX / sp1.L));
-            t.w2 = std::max(1, std::min(sp1.L, RG_SMOOTH_MAX / sp2.L));
+            auto balanced = [](int len, int wmax) {  // near-equal tile widths <= wmax
+                wmax = std::max(1, std::min(len, wmax));
+                const int tiles = (len + wmax - 1) / wmax;
+                return (len + tiles - 1) / tiles;
+            };
+            t.w1 = balanced(sp2.L, RG_SMOOTH_MAX / sp1.L);
+            t.w2 = balanced(sp1.L, RG_SMOOTH_MAX / sp2.L);
             t.tw_a = phase_table(prec, sp1.L, 1, 0, sp1.L);
             t.tw_b = phase_table(prec, sp2.L, 1, 0, sp2.L);
             if (!t.tw_a || !t.tw_b || !tw2_table(n, &t.tw2)) { ragged_free(r); return NFFT_E_HIP; }
@@ -681,7 +686,7 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
             // for M <= line limit (natural) or M1 x M2 (permuted); the ragged
             // kernel needs natural order for M <= 4096 and the [k1][k2] order
             // for its own M1 x M2 split, so H is rebuilt here when they differ.
-            if (!e.init(n, prec, false)) { ragged_free(r); return NFFT_E_HIP; }
+            if (!e.init(n, prec, 0)) { ragged_free(r); return NFFT_E_HIP; }
             t.chirp = e.chirp;
             size_kind[n] = m <= 4096 ? RG_BLUE1 : RG_BLUE3;
             if (m <= 4096) {

@@ -243,9 +243,10 @@ SMOOTH = [6, 9, 12, 15, 25, 27, 45, 49, 96, 100, 210, 243, 343, 625, 1000, 1715,
 
 
 @pytest.mark.parametrize("prec,n", [("f32", n) for n in SMOOTH] + [("f64", n) for n in SMOOTH if n <= 4096])
-def test_mixed_radix(pkg, dev, prec, n):
+def test_mixed_radix(pkg, dev, prec, n, monkeypatch):
     """7-smooth lengths run as mixed-radix lines (several lines per
     workgroup, a ragged last workgroup): vs pocketfft, and the plan says so."""
+    monkeypatch.setenv("NFFT_SMOOTH", "2")  # mixed radix wherever possible
     batch = 7
     x = rounded(ol.uniform(11, 2 * n * batch), prec)
     hi = il(np.fft.fft(cx(x).reshape(batch, n), axis=1))
@@ -264,9 +265,10 @@ SMOOTH4 = [10000, 15625, 40000, 65610, 196608, 1000000, 531441]
 
 
 @pytest.mark.parametrize("prec,n", [("f32", n) for n in SMOOTH4] + [("f64", n) for n in SMOOTH4[:5]])
-def test_mixed_radix_four_step(pkg, dev, prec, n):
+def test_mixed_radix_four_step(pkg, dev, prec, n, monkeypatch):
     """7-smooth lengths above one workgroup: mixed-radix column pass with the
     inter-pass twiddle, then row pass with a transposed store."""
+    monkeypatch.setenv("NFFT_SMOOTH", "2")
     batch = 3 if n <= 200000 else 1
     x = rounded(ol.uniform(12, 2 * n * batch), prec)
     hi = il(np.fft.fft(cx(x).reshape(batch, n), axis=1))
@@ -279,3 +281,25 @@ def test_mixed_radix_four_step(pkg, dev, prec, n):
     p = pkg.Plan("dft", prec, [n], batch)
     assert p.describe().startswith("mixed-radix four-step"), p.describe()
     p.close()
+
+
+def test_smooth_autotuner(pkg, dev, tmp_path, monkeypatch):
+    """NFFT_TUNE=1 times mixed radix against Bluestein at plan creation and
+    records the winner in NFFT_TUNE_CACHE; later plans follow the record,
+    and both choices give the right answer."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("NFFT_TUNE_CACHE", str(cache))
+    monkeypatch.setenv("NFFT_TUNE", "1")
+    n, batch = 3000, 4
+    x = rounded(ol.uniform(13, 2 * n * batch), "f32")
+    want = il(np.fft.fft(cx(x).reshape(batch, n), axis=1))
+    got = run_plan(pkg, dev, "dft", [n], x, batch, "f32")
+    assert ol.rel_l2(got, want) < tol_for("f32", n)
+    lines = [l.split() for l in cache.read_text().splitlines()]
+    rec = [l for l in lines if l[:3] == ["c2c-smooth", "p0", f"n{n}"]]
+    assert len(rec) == 1 and rec[0][3] in ("0", "1"), lines
+    monkeypatch.setenv("NFFT_TUNE", "0")
+    p = pkg.Plan("dft", "f32", [n], batch)
+    assert p.describe().startswith("mixed-radix") == (rec[0][3] == "1"), p.describe()
+    p.close()
+    assert len(cache.read_text().splitlines()) == len(lines)  # no second measurement
