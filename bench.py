@@ -19,6 +19,7 @@ Also printed in the same JSON line:
   accuracy      rel-L2 of sampled output transforms vs the FP64 oracle
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-sweep] [--no-cpu] [--calib]
+                       [--only NAME[,NAME...]]
 """
 import argparse
 import json
@@ -136,7 +137,7 @@ def timed(torch, fn, steps=5, warmup=2):
     return (time.perf_counter() - t0) / steps
 
 
-def config_extras(pkg, torch):
+def config_extras(pkg, torch, only=None):
     """The other workloads of the configuration list, one timing each (FP32
     unless stated): R2C batches, 3D C2C 512^3, 2D R2C 4096^2, DCT-2/DST-2/DCT-4
     rows, the ragged batch of 10,000 mixed sizes (FP32 and FP64) and the
@@ -147,7 +148,12 @@ def config_extras(pkg, torch):
     dev = torch.device("cuda:0")
     out = {}
 
+    def wanted(name):
+        return not only or any(name.startswith(o) for o in only)
+
     def plan_run(name, kind, prec, dims, batch, op, nin, nout, flops, vjp=False):
+        if not wanted(name):
+            return
         dt = torch.float32 if prec == "f32" else torch.float64
         x = torch.empty(nin * batch, dtype=dt, device=dev)
         y = torch.empty(nout * batch, dtype=dt, device=dev)
@@ -187,6 +193,13 @@ def config_extras(pkg, torch):
     n = 1 << 14
     plan_run("adj_dct2_f32_N2^14_x1024_vjp", "t2t3", "f32", [n], 1024, "dct2", n, n, 2.5 * n * math.log2(n),
              vjp=True)
+    # 7-smooth lengths (mixed radix) beside the power-of-two sweep
+    for prec, n in (("f32", 100), ("f64", 100), ("f64", 10000), ("f32", 10000)):
+        batch = (1 << 27) // n
+        plan_run(f"mixed_radix_{prec}_N{n}_x{batch}", "dft", prec, [n], batch, "dft", 2 * n, 2 * n,
+                 5 * n * math.log2(n))
+    if not wanted("ragged"):
+        return out
     sizes = ragged_sizes(10000)
     lib = pkg.core()
     for prec in ("f32", "f64"):
@@ -336,7 +349,13 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--calib", action="store_true",
                     help="first run a plain device copy of 1 GiB (reference kernel for PMC byte counters)")
+    ap.add_argument("--only", default="",
+                    help="comma-separated configuration names (prefixes) to run after the headline; "
+                         "implies --no-sweep --no-cpu")
     args = ap.parse_args()
+    only = [o for o in args.only.split(",") if o]
+    if only:
+        args.no_sweep = args.no_cpu = True
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "batched 1D transforms shard by splitting the batch, with no exchange", file=sys.stderr)
@@ -396,8 +415,8 @@ def main():
                     f"(dominant {dom} {kern[dom]['gbs']:.0f} GB/s) rel-L2 {r['rel_l2']:.2e} [{r['plan']}]")
 
     extras = {}
-    if not args.no_sweep:
-        extras = config_extras(pkg, torch)
+    if not args.no_sweep or only:
+        extras = config_extras(pkg, torch, only)
 
     cpu = None
     if not args.no_cpu:
