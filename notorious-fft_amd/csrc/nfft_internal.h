@@ -87,6 +87,7 @@ hipError_t launch_blue_line(int prec, int M, const BlueArgs &a, hipStream_t s);
 hipError_t launch_col(int prec, int L, int dir, int mode, const ColArgs &a, hipStream_t s);
 hipError_t launch_rowT(int prec, int L, int dir, const RowTArgs &a, hipStream_t s);
 hipError_t launch_blue_row(int prec, int L, const BlueRowArgs &a, hipStream_t s);
+hipError_t launch_rowT_r2c(int prec, int L, const RowR2CArgs &a, hipStream_t s);
 hipError_t launch_r2c(int prec, int L, const RtArgs &a, hipStream_t s);
 hipError_t launch_c2r(int prec, int L, const RtArgs &a, hipStream_t s);
 hipError_t launch_t23(int prec, int L, int op, const RtArgs &a, hipStream_t s);
@@ -187,7 +188,9 @@ struct C2C {
     // smooth_mode for 7-smooth non-power-of-two n: 2 = choose (tuned /
     // measured rule), 1 = mixed radix, 0 = Bluestein (callers that need its
     // chirp / filter tables)
-    bool init(int n, int prec, int smooth_mode = 2);
+    // persist_mode = 0 keeps the two-launch four-step for powers of two
+    // (callers that fuse their own work into its row pass)
+    bool init(int n, int prec, int smooth_mode = 2, int persist_mode = 2);
     void release();
     std::string describe() const;
     // dir: -1 forward, +1 inverse (unnormalised); scale applied to the output.

@@ -4,5 +4,6 @@
 namespace nfft {
 template hipError_t col_impl<float>(int, int, int, const ColArgs &, hipStream_t);
 template hipError_t rowT_impl<float>(int, int, const RowTArgs &, hipStream_t);
+template hipError_t rowT_r2c_impl<float>(int, const RowR2CArgs &, hipStream_t);
 template hipError_t blue_row_impl<float>(int, const BlueRowArgs &, hipStream_t);
 }  // namespace nfft

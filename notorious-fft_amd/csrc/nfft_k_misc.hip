@@ -43,6 +43,12 @@ hipError_t launch_rowT(int prec, int L, int dir, const RowTArgs &a, hipStream_t 
     NFFT_TIMED("k_rowT", L, 2.0 * a.ntrans * a.nrows * L * cxb(prec),
                prec == F32 ? rowT_impl<float>(L, dir, a, s) : rowT_impl<double>(L, dir, a, s));
 }
+hipError_t launch_rowT_r2c(int prec, int L, const RowR2CArgs &a, hipStream_t s)
+{
+    NFFT_TIMED("k_rowT_r2c", L,
+               (double)a.ntrans * a.nrows * L * cxb(prec) + (double)a.ntrans * (a.nrows * (double)L + 1) * cxb(prec),
+               prec == F32 ? rowT_r2c_impl<float>(L, a, s) : rowT_r2c_impl<double>(L, a, s));
+}
 hipError_t launch_blue_row(int prec, int L, const BlueRowArgs &a, hipStream_t s)
 {
     NFFT_TIMED("k_blue_row", L, 2.0 * a.ntrans * a.nrows * L * cxb(prec),

@@ -485,3 +485,97 @@ template <typename T> __global__ void k_ew_n1(EwArgs a)
 }
 
 }  // namespace nfft
+
+namespace nfft {
+
+// ---------------------------------------------------------------------------
+// k_rowT_r2c: the row pass of the four-step for the N/2-point complex FFT of
+// a realdft, with the realdft unpack fused in.  Z (Lc = n1 * L points per
+// transform, row r = L contiguous points) -> row FFTs -> X[k] for k <= Lc
+// (FFTW layout, Lc + 1 complex).  Output k = k2 * n1 + r needs
+// Z[Lc - k], which lies in row n1 - r (column L - 1 - k2), or in row 0
+// (column -k2 mod L) for r = 0 and in row n1/2 itself for r = n1/2: so
+// each workgroup transforms row pairs (p, n1 - p), with pair 0 = rows
+// (0, n1/2), and unpacks from its own LDS.  Replaces the separate
+// elementwise unpack pass (one full read + write of the spectrum).
+// ---------------------------------------------------------------------------
+struct RowR2CArgs {
+    const void *in;    // Z: Lc complex per transform
+    void *out;         // X: out_dist complex per transform (>= Lc + 1)
+    int nrows;         // n1 (even)
+    long long in_dist, out_dist;
+    long long ntrans;
+    const void *tw;    // stage table of L
+    const void *w;     // e^{-2 pi i k / (2 Lc)}, k < Lc
+    double scale;
+    int dbl;           // adjoint of invrealdft: bins 0 < k < Lc times 2
+};
+
+template <typename T, int L, int LPB>
+__global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT_r2c(RowR2CArgs a)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    static_assert(LPB % 2 == 0, "row pairs");
+    constexpr int NTH = S::NT * LPB, PADL = lpadlen<T, L>();
+    __shared__ C lds[LPB * PADL];
+    const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
+    const long long lin = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
+    const long long lg = xcd_logical(lin, (long long)gridDim.x * gridDim.y * gridDim.z);
+    const int pt = (int)(lg % gridDim.x);
+    const long long t = lg / gridDim.x;
+    if (t >= a.ntrans) return;  // uniform per workgroup
+    const int n1 = a.nrows, npairs = n1 / 2;
+    // row of a slot (-1: past the last pair)
+    auto row_of = [&](int slot) {
+        const int p = pt * (LPB / 2) + slot / 2;
+        if (p >= npairs) return -1;
+        return (slot & 1) ? (p == 0 ? npairs : n1 - p) : p;
+    };
+    const C *src = (const C *)a.in + t * a.in_dist;
+    C reg[S::R];
+    if constexpr (S::NT >= 64) {
+        const int r = row_of(b);
+#pragma unroll
+        for (int q = 0; q < S::R; q++) reg[q] = r >= 0 ? src[(long long)r * L + j + S::NT * q] : czero<T>();
+    } else {
+        for (int idx = threadIdx.x; idx < LPB * L; idx += NTH) {
+            const int slot = idx / L, p = idx & (L - 1);
+            const int r = row_of(slot);
+            lds[slot * PADL + lpad<T>(p)] = r >= 0 ? src[(long long)r * L + p] : czero<T>();
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < S::R; q++) reg[q] = lds[b * PADL + lpad<T>(j + S::NT * q)];
+        __syncthreads();
+    }
+    LayLine<T, L> lay{b * PADL};
+    line_fft<T, L, -1>(reg, lds, lay, j, (const double2 *)a.tw);
+#pragma unroll
+    for (int q = 0; q < S::R; q++) lds[b * PADL + lpad<T>(j + S::NT * q)] = reg[q];
+    __syncthreads();
+    const long long Lc = (long long)n1 * L;
+    C *dst = (C *)a.out + t * a.out_dist;
+    const C *w = (const C *)a.w;
+    const T sc = (T)a.scale, sk = a.dbl ? (T)2 * sc : sc;
+    for (int idx = threadIdx.x; idx < LPB * L; idx += NTH) {
+        const int slot = idx % LPB, k2 = idx / LPB;
+        const int r = row_of(slot);
+        if (r < 0) continue;
+        int ps = slot ^ 1, pc = L - 1 - k2;
+        if (r == 0) {
+            ps = slot;
+            pc = (L - k2) & (L - 1);
+        } else if (r == npairs) {
+            ps = slot;
+        }
+        const C zk = lds[slot * PADL + lpad<T>(k2)], zm = lds[ps * PADL + lpad<T>(pc)];
+        const long long k = (long long)k2 * n1 + r;
+        C xk, xl;
+        unpack_pair(zk, zm, w[k], xk, xl);
+        dst[k] = cscale(xk, k == 0 ? sc : sk);
+        if (k == 0) dst[Lc] = cscale(xl, sc);
+    }
+}
+
+}  // namespace nfft

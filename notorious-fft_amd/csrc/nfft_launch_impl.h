@@ -171,6 +171,31 @@ template <typename T> hipError_t rowT_impl(int L, int dir, const RowTArgs &a, hi
     return hipErrorInvalidValue;
 }
 
+template <typename T, int L> constexpr bool rowr2c_fits()
+{
+    return rowT_fits<T, L>() && rowT_lpb<T, L>() % 2 == 0;
+}
+
+template <typename T> hipError_t rowT_r2c_impl(int L, const RowR2CArgs &a, hipStream_t s)
+{
+    switch (L) {
+#define NFFT_CASE(LL)                                                                                         \
+    case LL:                                                                                                  \
+        if constexpr (rowr2c_fits<T, LL>()) {                                                                 \
+            constexpr int LPB = rowT_lpb<T, LL>();                                                            \
+            unsigned gy, gz;                                                                                  \
+            grid_trans(a.ntrans, gy, gz);                                                                     \
+            dim3 g((unsigned)((a.nrows / 2 + LPB / 2 - 1) / (LPB / 2)), gy, gz), b(Sched<LL>::NT * LPB);      \
+            hipLaunchKernelGGL((k_rowT_r2c<T, LL, LPB>), g, b, 0, s, a);                                      \
+            return hipGetLastError();                                                                         \
+        }                                                                                                     \
+        break;
+        NFFT_POW2_LIST(NFFT_CASE)
+#undef NFFT_CASE
+    }
+    return hipErrorInvalidValue;
+}
+
 template <typename T, int L> constexpr bool bluerow_fits()
 {
     constexpr int lpb = bluerow_lpb<T, L>();
@@ -272,6 +297,8 @@ extern template hipError_t col_impl<float>(int, int, int, const ColArgs &, hipSt
 extern template hipError_t col_impl<double>(int, int, int, const ColArgs &, hipStream_t);
 extern template hipError_t rowT_impl<float>(int, int, const RowTArgs &, hipStream_t);
 extern template hipError_t rowT_impl<double>(int, int, const RowTArgs &, hipStream_t);
+extern template hipError_t rowT_r2c_impl<float>(int, const RowR2CArgs &, hipStream_t);
+extern template hipError_t rowT_r2c_impl<double>(int, const RowR2CArgs &, hipStream_t);
 extern template hipError_t blue_row_impl<float>(int, const BlueRowArgs &, hipStream_t);
 extern template hipError_t blue_row_impl<double>(int, const BlueRowArgs &, hipStream_t);
 extern template hipError_t r2c_impl<float>(int, bool, const RtArgs &, hipStream_t);

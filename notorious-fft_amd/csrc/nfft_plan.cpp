@@ -359,7 +359,7 @@ static bool persist_setup(C2C &c)
     return true;
 }
 
-bool C2C::init(int n_, int prec_, int smooth_mode)
+bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
 {
     n = n_;
     prec = prec_;
@@ -376,7 +376,7 @@ bool C2C::init(int n_, int prec_, int smooth_mode)
     if (is_pow2(n)) {
         algo = ALG_FOUR;
         scratch_per_line = n;
-        if (env_int("NFFT_PERSIST", persist_default(prec, n) ? 1 : 0) && persist_find(prec, n, env_int("NFFT_PERSIST_VAR", 0), &pcfg)) {
+        if (persist_mode != 0 && env_int("NFFT_PERSIST", persist_default(prec, n) ? 1 : 0) && persist_find(prec, n, env_int("NFFT_PERSIST_VAR", 0), &pcfg)) {
             n1 = pcfg.n1;
             n2 = pcfg.n2;
             if (!persist_setup(*this)) return false;
@@ -732,7 +732,9 @@ bool R2C::init(int N_, int prec_)
         return tw != nullptr;
     }
     single = false;
-    if (!inner.init(L, prec)) return false;
+    // the row pass of a two-launch four-step absorbs the unpack (k_rowT_r2c)
+    if (!inner.init(L, prec, 2, env_int("NFFT_R2C_FUSE", 1) ? 0 : 2)) return false;
+    fused = inner.algo == ALG_FOUR && !inner.pctr && (inner.n1 % 2) == 0 && env_int("NFFT_R2C_FUSE", 1) != 0;
     scratch_per_line = L + inner.scratch_per_line;
     return true;
 }
@@ -758,10 +760,33 @@ hipError_t R2C::fwd(const void *in, void *out, long long nlines, long long in_di
     void *isc = (char *)scratch + (size_t)chunk * L * cb;
     for (long long l0 = 0; l0 < nlines; l0 += chunk) {
         const long long cnt = std::min(chunk, nlines - l0);
-        hipError_t e = inner.run((const char *)in + (size_t)l0 * in_dist * rb, Z, cnt, in_dist / 2, L, -1, 1.0, isc,
-                                 chunk, s);
+        const void *src = (const char *)in + (size_t)l0 * in_dist * rb;
+        void *dst = (char *)out + (size_t)l0 * out_dist * cb;
+        hipError_t e;
+        if (fused) {
+            // column pass (+ inter-pass twiddle) into Z, then rows + unpack
+            ColArgs c{};
+            c.in = src;
+            c.out = Z;
+            c.ncols = inner.n2;
+            c.S = inner.n2;
+            c.in_dist = in_dist / 2;
+            c.out_dist = L;
+            c.ntrans = cnt;
+            c.tw = inner.tw_a;
+            c.tw2 = inner.tw2;
+            c.scale = 1.0;
+            c.lpb = inner.col_lpb;
+            e = launch_col(prec, inner.n1, -1, COL_TWIDDLE, c, s);
+            if (e != hipSuccess) return e;
+            RowR2CArgs r{Z, dst, inner.n1, L, out_dist, cnt, inner.tw_b, rtw, scale, dbl};
+            e = launch_rowT_r2c(prec, inner.n2, r, s);
+            if (e != hipSuccess) return e;
+            continue;
+        }
+        e = inner.run(src, Z, cnt, in_dist / 2, L, -1, 1.0, isc, chunk, s);
         if (e != hipSuccess) return e;
-        EwArgs a{Z, (char *)out + (size_t)l0 * out_dist * cb, cnt, L, out_dist, L, rtw, nullptr, scale, 1.0, dbl, 0};
+        EwArgs a{Z, dst, cnt, L, out_dist, L, rtw, nullptr, scale, 1.0, dbl, 0};
         e = launch_ew(prec, EW_UNPACK, a, L, s);
         if (e != hipSuccess) return e;
     }
@@ -993,7 +1018,9 @@ bool nfft_plan::init(int kind_, int prec_, int d_, const int *dims_, long long b
         work_bytes = (size_t)(full + eng) * cb;
     } else if (kind == NFFT_KIND_REALDFT) {
         if (!real.init(last, prec)) return false;
-        dsc = "realdft " + std::to_string(last) + (real.single ? " single-pass" : " via " + real.inner.describe());
+        dsc = "realdft " + std::to_string(last) +
+              (real.single ? " single-pass"
+                           : " via " + real.inner.describe() + (real.fused ? " (unpack fused into the row pass)" : ""));
         const long long H = last / 2 + 1;
         const long long Pl = P / last;  // leading product
         ax.resize(d - 1);

@@ -9,13 +9,15 @@ from gpu_util import rounded, run_plan, tol_for
 
 pytestmark = pytest.mark.gpu
 
-R32 = [1, 2, 4, 8, 16, 64, 256, 1024, 4096, 16384, 32768, 1 << 16, 1 << 17]
-R64 = [1, 2, 4, 16, 1024, 8192, 16384, 1 << 15]
+R32 = [1, 2, 4, 8, 16, 64, 256, 1024, 4096, 16384, 32768, 1 << 16, 1 << 17, 1 << 20, 1 << 21]
+R64 = [1, 2, 4, 16, 1024, 8192, 16384, 1 << 15, 1 << 17, 1 << 20]
 
 
 @pytest.mark.parametrize("prec,n", [("f32", n) for n in R32] + [("f64", n) for n in R64])
 def test_realdft(pkg, dev, prec, n):
-    batch = 3
+    """Single-pass lines, and the four-step with the unpack fused into its
+    row pass (row pairs p, n1 - p) from N = 2^16 (FP32) / 2^15 (FP64)."""
+    batch = 3 if n <= (1 << 17) else 2
     x = rounded(ol.uniform(21, n * batch), prec)
     got = run_plan(pkg, dev, "realdft", [n], x, batch, prec)
     ref = ol.run_oracle("realdft", [n], x, batch)
