@@ -193,6 +193,49 @@ def config_extras(pkg, torch, only=None):
     n = 1 << 14
     plan_run("adj_dct2_f32_N2^14_x1024_vjp", "t2t3", "f32", [n], 1024, "dct2", n, n, 2.5 * n * math.log2(n),
              vjp=True)
+    # the drop-in C API on host buffers (staged through the aux's device
+    # buffers, so PCIe is in the loop): configuration C1 (FP64 N = 4096,
+    # dft then invdft, seed 0) and one large FP32 transform
+    if wanted("dropin"):
+        import numpy as np
+        import oracle_lib as ol
+        lib64 = pkg.dropin("f64")
+        n = 4096
+        a = lib64.notorious_fft_mkaux_dft_1d(n)
+        x = ol.uniform(0, 2 * n)
+        y = np.zeros_like(x)
+        z = np.zeros_like(x)
+
+        def c1():
+            lib64.notorious_fft_dft(x.ctypes.data, y.ctypes.data, a)
+            lib64.notorious_fft_invdft(y.ctypes.data, z.ctypes.data, a)
+        for _ in range(5):
+            c1()
+        t0 = time.perf_counter()
+        reps = 200
+        for _ in range(reps):
+            c1()
+        t = (time.perf_counter() - t0) / reps
+        rt = ol.rel_l2(z, n * x)
+        lib64.notorious_fft_free_aux(a)
+        out["dropin_c1_f64_N4096_dft_invdft_host"] = {"us_per_round_trip": round(1e6 * t, 1),
+                                                       "round_trip_rel_l2": float(f"{rt:.3e}")}
+        log(f"  dropin C1: {out['dropin_c1_f64_N4096_dft_invdft_host']}")
+        lib32 = pkg.dropin("f32")
+        n = 1 << 22
+        a = lib32.notorious_fft_mkaux_dft_1d(n)
+        x = np.asarray(ol.uniform(0, 2 * n), dtype=np.float32)
+        y = np.zeros_like(x)
+        for _ in range(2):
+            lib32.notorious_fft_dft(x.ctypes.data, y.ctypes.data, a)
+        t0 = time.perf_counter()
+        for _ in range(10):
+            lib32.notorious_fft_dft(x.ctypes.data, y.ctypes.data, a)
+        t = (time.perf_counter() - t0) / 10
+        lib32.notorious_fft_free_aux(a)
+        out["dropin_f32_N2^22_host"] = {"ms": round(1e3 * t, 3), "pcie_inclusive_gbs": round(2 * n * 8 / t / 1e9, 2)}
+        log(f"  dropin host 2^22: {out['dropin_f32_N2^22_host']}")
+
     # 7-smooth lengths (mixed radix) beside the power-of-two sweep
     for prec, n in (("f32", 100), ("f64", 100), ("f64", 10000), ("f32", 10000)):
         batch = (1 << 27) // n
