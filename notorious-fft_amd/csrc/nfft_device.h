@@ -243,10 +243,9 @@ template <int DIR, typename C> __device__ __forceinline__ C ctw64(C a, double2 w
 // FP32 lines round the four bases once and form the powers in FP32 (a few
 // FP32 ulps per twiddle, at half the cost of FP64 products and without the
 // per-twiddle FP64 -> FP32 conversions).
-template <int DIR> __device__ __forceinline__ void apply_stage_tw32(Cx<float> *a, const double2 (&b)[4])
+template <int DIR> __device__ __forceinline__ void apply_stage_tw32(Cx<float> *a, const float2 (&b)[4])
 {
-    const Cx<float> w1 = to_cx<float>(b[0]), w2 = to_cx<float>(b[1]), w4 = to_cx<float>(b[2]),
-                    w8 = to_cx<float>(b[3]);
+    const Cx<float> w1 = b[0], w2 = b[1], w4 = b[2], w8 = b[3];
     const Cx<float> w3 = cmul(w2, w1), w5 = cmul(w4, w1), w6 = cmul(w4, w2), w7 = cmul(w4, w3);
     a[1] = ctw<DIR>(a[1], w1);
     a[2] = ctw<DIR>(a[2], w2);
@@ -265,10 +264,15 @@ template <int DIR> __device__ __forceinline__ void apply_stage_tw32(Cx<float> *a
     a[15] = ctw<DIR>(a[15], cmul(w8, w7));
 }
 
+template <int DIR> __device__ __forceinline__ void apply_stage_tw(Cx<float> *a, const float2 (&b)[4])
+{
+    apply_stage_tw32<DIR>(a, b);
+}
 template <int DIR, typename C> __device__ __forceinline__ void apply_stage_tw(C *a, const double2 (&b)[4])
 {
     if constexpr (sizeof(a[0].x) == 4) {
-        apply_stage_tw32<DIR>(a, b);
+        const float2 f[4] = {to_cx<float>(b[0]), to_cx<float>(b[1]), to_cx<float>(b[2]), to_cx<float>(b[3])};
+        apply_stage_tw32<DIR>(a, f);
         return;
     }
     // ordered so that at most two FP64 temporaries are live at a time
@@ -299,27 +303,57 @@ template <int DIR, typename C> __device__ __forceinline__ void apply_stage_tw(C 
     }
 }
 
+// Where the line FFT takes the base powers [w, w^2, w^4, w^8] of its
+// twiddled stages: the FP64 table in global memory (requested one stage
+// ahead), or a copy in LDS (StageTwLds, Cx<T> entries: FP32 kernels store the
+// bases rounded once, exactly the values they would convert).  The LDS copy
+// keeps a kernel's FFT free of global loads, so loads it issued for its next
+// tile stay in flight through the whole transform.
+struct StageTwGlobal {
+    const double2 *tw;
+    using B = double2;
+    static constexpr bool kAhead = true;  // global latency: fetch one stage early
+    __device__ __forceinline__ void load(int off, B (&bw)[4]) const
+    {
+#pragma unroll
+        for (int e = 0; e < 4; e++) bw[e] = tw[off + e];
+    }
+};
+template <typename T> struct StageTwLds {
+    const Cx<T> *tw;  // LDS
+    using B = Cx<T>;
+    static constexpr bool kAhead = false;
+    __device__ __forceinline__ void load(int off, B (&bw)[4]) const
+    {
+#pragma unroll
+        for (int e = 0; e < 4; e++) bw[e] = tw[off + e];
+    }
+};
+// Copy the stage bases of length L into LDS (all threads of the block take
+// part; the caller synchronises before use).
+template <typename T, int L> __device__ __forceinline__ void stage_tw_to_lds(Cx<T> *dst, const double2 *tw)
+{
+    for (int i = threadIdx.x; i < Sched<L>::TWN; i += blockDim.x) dst[i] = to_cx<T>(tw[i]);
+}
+
 // ---------------------------------------------------------------------------
 // The line FFT.  All threads of the block must call it (it contains block
-// barriers when S > 1).  reg[q] <-> point j + NT*q.  The base twiddles of
-// stage s+1 are requested at the start of stage s so that their latency
+// barriers when S > 1).  reg[q] <-> point j + NT*q.  With global bases, those
+// of stage s+1 are requested at the start of stage s so that their latency
 // overlaps the butterflies and the LDS exchange.
 // ---------------------------------------------------------------------------
-template <typename T, int L, int DIR, class Lay>
-__device__ __forceinline__ void line_fft(Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds, const Lay &lay, int j,
-                                         const double2 *__restrict__ tw)
+template <typename T, int L, int DIR, class Lay, class Tw>
+__device__ __forceinline__ void line_fft_tw(Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds, const Lay &lay, int j,
+                                            const Tw &twsrc)
 {
     using S = Sched<L>;
     using C = Cx<T>;
-    double2 bw[4];  // base powers of the current / next twiddled stage
+    typename Tw::B bw[4];  // base powers of the current / next twiddled stage
     sfor<0, S::S>([&](auto sc) {
         constexpr int s = decltype(sc)::value;
         constexpr int Rs = S::radix(s), Ns = S::ns(s), m = S::R / Rs;
-        if constexpr (s == 0 && S::S > 1) {
-            const double2 *t = tw + S::twoff(1) + (j & (S::ns(1) - 1)) * 4;
-#pragma unroll
-            for (int e = 0; e < 4; e++) bw[e] = t[e];
-        }
+        if constexpr (s == 0 && S::S > 1 && Tw::kAhead) twsrc.load(S::twoff(1) + (j & (S::ns(1) - 1)) * 4, bw);
+        if constexpr (s > 0 && !Tw::kAhead) twsrc.load(S::twoff(s) + (j & (S::ns(s) - 1)) * 4, bw);
 #pragma unroll
         for (int i = 0; i < m; i++) {
             C a[Rs];
@@ -334,12 +368,8 @@ __device__ __forceinline__ void line_fft(Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds, 
             for (int r = 0; r < Rs; r++) reg[i + r * m] = a[r];
         }
         if constexpr (s + 1 < S::S) {
-            if constexpr (s > 0) {
-                // request the next stage's bases; the exchange hides the latency
-                const double2 *t = tw + S::twoff(s + 1) + (j & (S::ns(s + 1) - 1)) * 4;
-#pragma unroll
-                for (int e = 0; e < 4; e++) bw[e] = t[e];
-            }
+            // request the next stage's bases; the exchange hides the latency
+            if constexpr (s > 0 && Tw::kAhead) twsrc.load(S::twoff(s + 1) + (j & (S::ns(s + 1) - 1)) * 4, bw);
 #pragma unroll
             for (int i = 0; i < m; i++) {
                 const int v = j + S::NT * i;
@@ -353,6 +383,12 @@ __device__ __forceinline__ void line_fft(Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds, 
             __syncthreads();
         }
     });
+}
+template <typename T, int L, int DIR, class Lay>
+__device__ __forceinline__ void line_fft(Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds, const Lay &lay, int j,
+                                         const double2 *__restrict__ tw)
+{
+    line_fft_tw<T, L, DIR>(reg, lds, lay, j, StageTwGlobal{tw});
 }
 
 // Twiddle for the four-step / Bluestein inter-pass products: w_N^{e},
@@ -377,10 +413,8 @@ __device__ __forceinline__ double2 tw2_get(const Tw2 &t, uint32_t e)
 // round them once and run two interleaved FP32 chains of R/2 products each
 // (error ~R/2 FP32 ulps on the last twiddles).
 template <int DIR, typename C, int R>
-__device__ __forceinline__ void apply_tw2_series(C (&reg)[R], const Tw2 &t, uint32_t base_e, uint32_t step_e)
+__device__ __forceinline__ void apply_tw2_from(C (&reg)[R], double2 w, const double2 st)
 {
-    double2 w = tw2_get(t, base_e);
-    const double2 st = tw2_get(t, step_e);
     if constexpr (sizeof(reg[0].x) == 4 && R % 2 == 0) {
         Cx<float> e = to_cx<float>(w), o = to_cx<float>(dmul(w, st));
         const Cx<float> st2 = to_cx<float>(dmul(st, st));
@@ -400,6 +434,68 @@ __device__ __forceinline__ void apply_tw2_series(C (&reg)[R], const Tw2 &t, uint
             if (q + 1 < R) w = dmul(w, st);
         }
     }
+}
+
+template <int DIR, typename C, int R>
+__device__ __forceinline__ void apply_tw2_series(C (&reg)[R], const Tw2 &t, uint32_t base_e, uint32_t step_e)
+{
+    apply_tw2_from<DIR>(reg, tw2_get(t, base_e), tw2_get(t, step_e));
+}
+
+// Global access as a workgroup-uniform base plus a 32-bit per-lane byte
+// offset: the compiler then addresses with the base in scalar registers
+// instead of 64-bit per-lane arithmetic on every access.  Callers take this
+// form only when off_fits(largest lane offset in elements) holds.
+template <typename C> __device__ __forceinline__ bool off_fits(long long max_elems)
+{
+    return max_elems >= 0 && max_elems * (long long)sizeof(C) < (1LL << 32);
+}
+template <typename C> __device__ __forceinline__ C ld_off(const C *base, uint32_t off)
+{
+    return *(const C *)((const char *)base + off);
+}
+template <typename C> __device__ __forceinline__ void st_off(C *base, uint32_t off, C v)
+{
+    *(C *)((char *)base + off) = v;
+}
+
+// Raw buffer access (gfx9 resource, no stride, range check wide open):
+// address = base + soff (scalar register, one per access) + voff (the lane's
+// 32-bit byte offset).  Strided tiles use it so that each access costs no
+// per-lane address arithmetic; callers keep soff + voff below 2^32.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, -1, 0x00020000);
+}
+// The same view, or (live == false) one with no records: every load through
+// it returns zero without a memory access.  Lets a prefetch be issued
+// unconditionally (no branch, so the wait counts after it stay exact).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc_if(const void *base, bool live)
+{
+    // the arguments are workgroup-uniform: pin them to scalar registers so
+    // that the descriptor needs no per-lane handling
+    const uint64_t p = (uint64_t)base;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)p);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(p >> 32));
+    const int n = __builtin_amdgcn_readfirstlane(live ? -1 : 0);
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(((uint64_t)hi << 32) | lo), (short)0, n, 0x00020000);
+}
+template <typename C> __device__ __forceinline__ C buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+{
+    if constexpr (sizeof(C) == 8)
+        return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+    else
+        return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+}
+template <typename C>
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, C v)
+{
+    if constexpr (sizeof(C) == 8)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
+                                              (int)voff, (int)soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v),
+                                               r, (int)voff, (int)soff, 0);
 }
 
 // Logical workgroup index for XCD-aware tiling: workgroups are dispatched

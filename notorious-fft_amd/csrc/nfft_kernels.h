@@ -70,11 +70,18 @@ __device__ __forceinline__ void tile_load(Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds,
     using S = Sched<L>;
     constexpr int NTH = S::NT * LPB;
     if constexpr (S::NT >= 64) {
-        const Cx<T> *s = src + b * dist;
+        if (nrows == LPB && len == L && off_fits<Cx<T>>((long long)(LPB - 1) * dist + L)) {
+            // full tile (uniform): unpredicated loads off one base
+            const uint32_t off = (uint32_t)(b * dist + j) * (uint32_t)sizeof(Cx<T>);
 #pragma unroll
-        for (int q = 0; q < S::R; q++) {
-            const int p = j + S::NT * q;
-            reg[q] = (b < nrows && p < len) ? s[p] : czero<T>();
+            for (int q = 0; q < S::R; q++) reg[q] = ld_off(src + S::NT * q, off);
+        } else {
+            const Cx<T> *s = src + b * dist;
+#pragma unroll
+            for (int q = 0; q < S::R; q++) {
+                const int p = j + S::NT * q;
+                reg[q] = (b < nrows && p < len) ? s[p] : czero<T>();
+            }
         }
     } else {
         stage_in_rows<T, L, LPB, NTH>(lds, src, dist, len, nrows);
@@ -93,7 +100,11 @@ __device__ __forceinline__ void tile_store(const Cx<T> (&reg)[Sched<L>::R], Cx<T
     using S = Sched<L>;
     constexpr int NTH = S::NT * LPB;
     if constexpr (S::NT >= 64) {
-        if (b < nrows) {
+        if (nrows == LPB && len == L && off_fits<Cx<T>>((long long)(LPB - 1) * dist + L)) {
+            const uint32_t off = (uint32_t)(b * dist + j) * (uint32_t)sizeof(Cx<T>);
+#pragma unroll
+            for (int q = 0; q < S::R; q++) st_off(dst + S::NT * q, off, reg[q]);
+        } else if (b < nrows) {
             Cx<T> *d = dst + b * dist;
 #pragma unroll
             for (int q = 0; q < S::R; q++) {
@@ -204,9 +215,13 @@ k_col(ColArgs a)
     const int b = threadIdx.x % LPB, j = threadIdx.x / LPB;
     const long long lin = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
     const long long lg = xcd_logical(lin, (long long)gridDim.x * gridDim.y * gridDim.z);
-    const int c = (int)(lg % gridDim.x) * LPB + b;
+    const int c0 = (int)(lg % gridDim.x) * LPB, c = c0 + b;
     const long long t = lg / gridDim.x;
     const bool valid = c < a.ncols && t < a.ntrans;
+    // uniform: every column of the tile exists and tile offsets fit 32 bits
+    const bool full = c0 + LPB <= a.ncols && t < a.ntrans && off_fits<C>((long long)L * a.S);
+    const uint32_t qst = (uint32_t)(S::NT * a.S) * (uint32_t)sizeof(C);  // byte step of q
+    const uint32_t loff = (uint32_t)(j * a.S + b) * (uint32_t)sizeof(C);
     C reg[S::R];
     if constexpr (MODE == COL_BLUE_IN) {
         const C *src = (const C *)a.in + t * a.in_dist;
@@ -222,6 +237,11 @@ k_col(ColArgs a)
             }
             reg[q] = v;
         }
+    } else if (full) {
+        // whole tile in range: unpredicated loads, uniform base per q
+        const auto rs = buf_rsrc((const C *)a.in + t * a.in_dist + c0);
+#pragma unroll
+        for (int q = 0; q < S::R; q++) reg[q] = buf_ld<C>(rs, loff, q * qst);
     } else {
         const C *src = (const C *)a.in + t * a.in_dist + c;
 #pragma unroll
@@ -246,6 +266,15 @@ k_col(ColArgs a)
                 if (a.conj) v = cconj(v);
                 dst[n] = cscale(v, sc);
             }
+        }
+    } else if (full) {
+        const auto rd = buf_rsrc((C *)a.out + t * a.out_dist + c0);
+        const T sc = (T)a.scale;
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            C v = reg[q];
+            if (sc != (T)1) v = cscale(v, sc);
+            buf_st(rd, loff, q * qst, v);
         }
     } else {
         C *dst = (C *)a.out + t * a.out_dist + c;
@@ -274,6 +303,7 @@ struct RowTArgs {
     const void *tw;
     double scale;
     int lpb;                      // rows per workgroup (0: default for L)
+    int pf;                       // full tiles through k_rowT_pf (resident, prefetching)
 };
 
 template <typename T, int L, int LPB> __host__ __device__ constexpr int rowT_lds()
@@ -311,9 +341,111 @@ k_rowT(RowTArgs a)
     }
     __syncthreads();
     C *dst = (C *)a.out + t * a.out_dist + r0;
+    constexpr int KS = NTH / LPB;  // output points k advanced per sweep
+    if (nrows == LPB && off_fits<C>((long long)L * a.nrows)) {
+        // full tile (uniform): L/KS independent LDS reads and stores per thread
+        const int bb = threadIdx.x % LPB, k0 = threadIdx.x / LPB;
+        const uint32_t off = (uint32_t)(k0 * a.nrows + bb) * (uint32_t)sizeof(C);
+        const uint32_t ist = (uint32_t)(KS * a.nrows) * (uint32_t)sizeof(C);
+        const auto rd = buf_rsrc(dst);
+#pragma unroll
+        for (int i = 0; i < L / KS; i++) buf_st(rd, off, i * ist, lds[(k0 + KS * i) * (LPB + 1) + bb]);
+        return;
+    }
     for (int idx = threadIdx.x; idx < LPB * L; idx += NTH) {
         const int bb = idx % LPB, k = idx / LPB;
         if (bb < nrows) dst[(long long)k * a.nrows + bb] = lds[k * (LPB + 1) + bb];
+    }
+}
+
+template <typename T, int L> __host__ __device__ constexpr int stage_tw_lds()
+{
+    return Sched<L>::TWN > 0 ? Sched<L>::TWN : 1;
+}
+
+// ---------------------------------------------------------------------------
+// k_rowT_pf: k_rowT over full tiles (nrows % LPB == 0) as a resident grid.
+// The tiles (tile = t * (nrows / LPB) + row group) are split into 8
+// contiguous shares, one per XCD (workgroup w runs on XCD w % 8); the
+// m = gridDim.x / 8 workgroups of an XCD walk its share m apart, so the
+// output segments written at any time are adjacent (the same DRAM pages).
+// Each workgroup issues the loads of its next tile before it transforms the
+// current one: with one or two workgroups per CU (the transpose fills the
+// LDS) the loads then overlap the FFT work instead of alternating with it.
+// The FFT reads its stage twiddles from an LDS copy, so no global load is
+// queued behind the prefetch.  Lanes read their points straight from global
+// memory (a wave covers 64 / NT rows of NT consecutive points: segments of
+// >= 128 bytes for L >= 256).  The host guarantees L * nrows elements and
+// ntiles fit the 32-bit offsets and indices used here.
+// ---------------------------------------------------------------------------
+template <typename T, int L, int LPB> __host__ __device__ constexpr int rowT_pf_lds()
+{
+    return rowT_lds<T, L, LPB>() + stage_tw_lds<T, L>();
+}
+
+template <typename T, int L, int LPB, int DIR>
+__global__ void __launch_bounds__(Sched<L>::NT *LPB,
+                                  (wpe_hint<T>(Sched<L>::NT * LPB, (long)rowT_pf_lds<T, L, LPB>() * sizeof(Cx<T>))))
+k_rowT_pf(RowTArgs a, long long ntiles)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    constexpr int NTH = S::NT * LPB;
+    constexpr int KS = NTH / LPB;  // output points k advanced per sweep
+    __shared__ C lds[rowT_lds<T, L, LPB>()];
+    __shared__ C ltw[stage_tw_lds<T, L>()];
+    const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
+    const unsigned gpt = (unsigned)(a.nrows / LPB);
+    const unsigned x = blockIdx.x & 7, m = gridDim.x >> 3;
+    const unsigned q8 = (unsigned)(ntiles >> 3), r8 = (unsigned)(ntiles & 7);
+    const unsigned lo = x * q8 + (x < r8 ? x : r8), hi = lo + q8 + (x < r8 ? 1 : 0);
+    const uint32_t loff = (uint32_t)(b * L + j) * (uint32_t)sizeof(C);
+    const int bb = threadIdx.x % LPB, k0 = threadIdx.x / LPB;
+    const uint32_t soff = (uint32_t)(k0 * a.nrows + bb) * (uint32_t)sizeof(C);
+    const uint32_t ist = (uint32_t)(KS * a.nrows) * (uint32_t)sizeof(C);
+    const T sc = (T)a.scale;
+    unsigned tl = lo + (blockIdx.x >> 3);
+    if (tl >= hi) return;  // uniform: no tile for this workgroup
+    stage_tw_to_lds<T, L>(ltw, (const double2 *)a.tw);
+    C reg[S::R], nxt[S::R];
+    {
+        const unsigned t = tl / gpt;
+        const auto rs = buf_rsrc_if((const C *)a.in + t * a.in_dist + (tl - t * gpt) * LPB * (long long)L, true);
+#pragma unroll
+        for (int q = 0; q < S::R; q++) reg[q] = buf_ld<C>(rs, loff, q * S::NT * (uint32_t)sizeof(C));
+    }
+    // Complete the first tile's loads here: the loop-top wait then covers
+    // only the prefetch, not the stores issued after it.
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    // Each iteration starts with its tile in reg[]; the hand-over from nxt[]
+    // sits at the end of the previous iteration, after its stores.
+    for (; tl < hi; tl += m) {
+        const unsigned t = tl / gpt;
+        const int r0 = (int)(tl - t * gpt) * LPB;
+        {
+            const unsigned tn = tl + m < hi ? tl + m : tl;
+            const unsigned u = tn / gpt;
+            const auto rs =
+                buf_rsrc_if((const C *)a.in + u * a.in_dist + (tn - u * gpt) * LPB * (long long)L, tl + m < hi);
+#pragma unroll
+            for (int q = 0; q < S::R; q++) nxt[q] = buf_ld<C>(rs, loff, q * S::NT * (uint32_t)sizeof(C));
+        }
+        LayLine<T, L> lay{b * lpadlen<T, L>()};
+        line_fft_tw<T, L, DIR>(reg, lds, lay, j, StageTwLds<T>{ltw});
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            C v = reg[q];
+            if (sc != (T)1) v = cscale(v, sc);
+            lds[(j + S::NT * q) * (LPB + 1) + b] = v;
+        }
+        __syncthreads();
+        const auto rd = buf_rsrc_if((C *)a.out + t * a.out_dist + r0, true);
+#pragma unroll
+        for (int i = 0; i < L / KS; i++) buf_st(rd, soff, i * ist, lds[(k0 + KS * i) * (LPB + 1) + bb]);
+#pragma unroll
+        for (int q = 0; q < S::R; q++) reg[q] = nxt[q];
+        __syncthreads();  // the next tile's exchanges overwrite the transpose
     }
 }
 

@@ -134,9 +134,44 @@ template <typename T, int L, int LPB> constexpr bool rowT_fits_v()
 }
 template <typename T, int L> constexpr bool rowT_fits() { return rowT_fits_v<T, L, rowT_lpb<T, L>()>(); }
 
+// The resident prefetching grid is used where it measured faster: rows of
+// 256..512 points, and longer rows with tiles of 16 or more rows (with 8-row
+// tiles of >= 1024 points the one-workgroup-per-tile pass was faster: FP32
+// 2048 x 8 961 vs 1084 us, 1024 x 8 899 vs 1118 us; FP64 1024 x 8 1581 vs
+// 1669 us, on 2^28 / 2^27 points).
+template <typename T, int L, int LPB> constexpr bool rowT_pf_fits()
+{
+    return Sched<L>::S > 1 && L >= 256 && (L <= 512 || LPB >= 16) &&
+           (size_t)rowT_pf_lds<T, L, LPB>() * sizeof(Cx<T>) <= kLdsLimit;
+}
+
+template <typename T, int L, int LPB, int DIR> bool rowT_pf_try(const RowTArgs &a, hipStream_t s)
+{
+    if (a.nrows % LPB != 0 || (long long)L * a.nrows * (long long)sizeof(Cx<T>) >= (1LL << 32)) return false;
+    static const long long resident = [] {
+        int dev = 0, ncu = 0, n = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_rowT_pf<T, L, LPB, DIR>,
+                                                         Sched<L>::NT * LPB, 0) != hipSuccess)
+            return 0LL;
+        return (long long)n * ncu;
+    }();
+    const long long ntiles = a.ntrans * (a.nrows / LPB);
+    if (ntiles >= (1LL << 31)) return false;  // 32-bit tile indices
+    const long long g = std::min(resident, (ntiles + 7) / 8 * 8) / 8 * 8;
+    if (g < 8) return false;
+    hipLaunchKernelGGL((k_rowT_pf<T, L, LPB, DIR>), dim3((unsigned)g), dim3(Sched<L>::NT * LPB), 0, s, a, ntiles);
+    return true;
+}
+
 template <typename T, int L, int LPB> hipError_t rowT_launch(int dir, const RowTArgs &a, hipStream_t s)
 {
     if constexpr (rowT_fits_v<T, L, LPB>()) {
+        if constexpr (rowT_pf_fits<T, L, LPB>()) {
+            if (a.pf && (dir < 0 ? rowT_pf_try<T, L, LPB, -1>(a, s) : rowT_pf_try<T, L, LPB, 1>(a, s)))
+                return hipGetLastError();
+        }
         unsigned gy, gz;
         grid_trans(a.ntrans, gy, gz);
         dim3 g((unsigned)((a.nrows + LPB - 1) / LPB), gy, gz), b(Sched<L>::NT * LPB);

@@ -395,6 +395,7 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
             }
             col_lpb = env_int("NFFT_COL_LPB", 0);
             row_lpb = env_int("NFFT_ROWT_LPB", 0);
+            row_pf = env_int("NFFT_ROWT_PF", 1) != 0 ? 1 : 0;
         }
         tw_a = stage_table(prec, n1);
         tw_b = stage_table(prec, n2);
@@ -621,7 +622,8 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
             c.lpb = col_lpb;
             hipError_t e = launch_col(prec, n1, dir, COL_TWIDDLE, c, s);
             if (e != hipSuccess) return e;
-            RowTArgs r{scratch, (char *)out + (size_t)l0 * out_dist * cb, n1, n, out_dist, cnt, tw_b, scale, row_lpb};
+            RowTArgs r{scratch, (char *)out + (size_t)l0 * out_dist * cb, n1, n, out_dist, cnt, tw_b, scale, row_lpb,
+                       row_pf};
             e = launch_rowT(prec, n2, dir, r, s);
             if (e != hipSuccess) return e;
         }

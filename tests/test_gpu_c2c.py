@@ -303,3 +303,34 @@ def test_smooth_autotuner(pkg, dev, tmp_path, monkeypatch):
     assert p.describe().startswith("mixed-radix") == (rec[0][3] == "1"), p.describe()
     p.close()
     assert len(cache.read_text().splitlines()) == len(lines)  # no second measurement
+
+
+ROWT_PF_CASES = [("f32", lg, 26) for lg in range(15, 23)] + [("f64", lg, 25) for lg in range(15, 22)]
+
+
+@pytest.mark.parametrize("prec,lg,lgtot", ROWT_PF_CASES)
+def test_rowT_prefetch_grid_bitwise(pkg, dev, prec, lg, lgtot, monkeypatch):
+    """The resident prefetching row pass against the one-workgroup-per-tile
+    row pass: the same arithmetic in the same order, so the outputs agree bit
+    for bit.  Batches of 2^lgtot points give every workgroup many tiles."""
+    import torch
+    n = 1 << lg
+    batch = 1 << (lgtot - lg)
+    dt = torch.float32 if prec == "f32" else torch.float64
+    x = torch.empty(2 * n * batch, dtype=dt, device=dev)
+    pkg.fill_uniform(x.data_ptr(), x.numel(), 11, 0, prec)
+    monkeypatch.setenv("NFFT_PERSIST", "0")
+    outs = {}
+    for pf in ("1", "0"):
+        monkeypatch.setenv("NFFT_ROWT_PF", pf)
+        plan = pkg.Plan("dft", prec, [n], batch)
+        assert "four-step" in plan.describe(), plan.describe()
+        for op in ("dft", "invdft"):
+            y = torch.empty_like(x)
+            plan.execute(op, x.data_ptr(), y.data_ptr())
+            outs[(pf, op)] = y
+        torch.cuda.synchronize()
+        plan.check()
+        plan.close()
+    for op in ("dft", "invdft"):
+        assert torch.equal(outs[("1", op)], outs[("0", op)]), (op, (outs[("1", op)] - outs[("0", op)]).abs().max())
