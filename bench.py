@@ -161,7 +161,10 @@ def config_extras(pkg, torch, only=None):
         p = pkg.Plan(kind, prec, dims, batch)
         fn = (lambda: p.vjp(op, x.data_ptr(), y.data_ptr())) if vjp else \
             (lambda: p.execute(op, x.data_ptr(), y.data_ptr()))
-        t = timed(torch, fn)
+        # 10 warmup calls: after host-side work (the oracle checks before
+        # these configurations) the GPU needs several milliseconds of load to
+        # return to full clocks; with 2 the timed calls ran up to 2x slow
+        t = timed(torch, fn, steps=20, warmup=10)
         eb = 4 if prec == "f32" else 8
         out[name] = {"ms": round(1e3 * t, 4), "gflops": round(flops * batch / t / 1e9, 1),
                      "alg_gbs": round((nin + nout) * batch * eb / t / 1e9, 1),
@@ -255,7 +258,8 @@ def config_extras(pkg, torch, only=None):
         x = torch.empty(2 * total, dtype=dt, device=dev)
         y = torch.empty_like(x)
         pkg.fill_uniform(x.data_ptr(), x.numel(), 0, 0, prec)
-        t = timed(torch, lambda: pkg.check(lib.nfft_ragged_execute(h, 0, x.data_ptr(), y.data_ptr(), None)))
+        t = timed(torch, lambda: pkg.check(lib.nfft_ragged_execute(h, 0, x.data_ptr(), y.data_ptr(), None)),
+                  steps=10, warmup=5)
         flops = sum(5.0 * m * math.log2(m) for m in sizes)
         eb = 8 if prec == "f32" else 16
         out[f"ragged_{prec}_10000"] = {"ms": round(1e3 * t, 4), "gflops": round(flops / t / 1e9, 1),
@@ -386,7 +390,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--n", type=int, default=1 << 20)

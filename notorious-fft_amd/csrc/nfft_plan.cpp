@@ -327,7 +327,10 @@ static const double kPersistTimeoutSec = 2.0;
 // Where the persistent four-step is the default (NFFT_PERSIST=0/1 forces it
 // off/on): the measured crossover on MI355X is FP32 N <= 2^17 and FP64
 // N <= 2^16; above it the two-launch path over whole-batch chunks is faster.
-static bool persist_default(int prec, long long n) { return n <= (prec == F32 ? (1LL << 17) : (1LL << 16)); }
+// Off by default: the two-launch passes with 1024-point rows measured as
+// fast or faster (FP32 2^15 1.53 vs 1.56 ms, 2^16 1.54 vs 1.55, 2^17 1.64 vs
+// 1.62; FP64 2^14..2^16 3.0 vs 3.5 ms); NFFT_PERSIST=1 selects it.
+static bool persist_default(int, long long) { return false; }
 // Intermediate the persistent kernel keeps in flight (slots), sized to stay
 // in the 256 MB Infinity Cache.
 static const long long kPersistSlotBytes = 128LL << 20;
@@ -381,8 +384,15 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
             n2 = pcfg.n2;
             if (!persist_setup(*this)) return false;
         } else {
+            // Rows of 1024 points (the prefetching row grid with 16-row tiles
+            // in FP32) over columns of 16..2048 points measured fastest at
+            // every length from 2^14 to 2^21, C2C and R2C, both precisions
+            // (2^28 points FP32 / 2^27 FP64: e.g. FP32 2^19 512x1024 1.64 ms
+            // vs 1024x512 1.80; FP64 2^16 64x1024 3.04 ms vs 256x256 3.15);
+            // beyond that the square split.
             const int lg = ilog2ll(n);
-            int lg2 = std::min(lg / 2, ilog2ll(rowT_max(prec)));
+            int lg2 = (lg - 10 >= 4 && lg - 10 <= 11) ? 10 : lg / 2;
+            lg2 = std::min(lg2, ilog2ll(rowT_max(prec)));
             const int forced = env_int("NFFT_FOUR_N2", 0);
             if (forced > 0 && is_pow2(forced) && forced <= rowT_max(prec) && n / forced <= col_max(prec) &&
                 n / forced >= 2)

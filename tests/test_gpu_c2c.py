@@ -305,14 +305,20 @@ def test_smooth_autotuner(pkg, dev, tmp_path, monkeypatch):
     assert len(cache.read_text().splitlines()) == len(lines)  # no second measurement
 
 
-ROWT_PF_CASES = [("f32", lg, 26) for lg in range(15, 23)] + [("f64", lg, 25) for lg in range(15, 22)]
+ROWT_PF_CASES = ([("f32", 16, n2, 26) for n2 in (256, 512, 1024, 2048)] +
+                 [("f32", lg, 0, 26) for lg in range(15, 23)] +
+                 [("f32", 20, n2, 26) for n2 in (512, 2048)] +
+                 [("f64", 16, n2, 25) for n2 in (256, 512, 1024)] +
+                 [("f64", lg, 0, 25) for lg in range(14, 22)] +
+                 [("f64", 19, 512, 25)])
 
 
-@pytest.mark.parametrize("prec,lg,lgtot", ROWT_PF_CASES)
-def test_rowT_prefetch_grid_bitwise(pkg, dev, prec, lg, lgtot, monkeypatch):
+@pytest.mark.parametrize("prec,lg,n2,lgtot", ROWT_PF_CASES)
+def test_rowT_prefetch_grid_bitwise(pkg, dev, prec, lg, n2, lgtot, monkeypatch):
     """The resident prefetching row pass against the one-workgroup-per-tile
     row pass: the same arithmetic in the same order, so the outputs agree bit
-    for bit.  Batches of 2^lgtot points give every workgroup many tiles."""
+    for bit.  Batches of 2^lgtot points give every workgroup many tiles; row
+    lengths n2 (0: the planner's split) cover every prefetching shape."""
     import torch
     n = 1 << lg
     batch = 1 << (lgtot - lg)
@@ -320,11 +326,15 @@ def test_rowT_prefetch_grid_bitwise(pkg, dev, prec, lg, lgtot, monkeypatch):
     x = torch.empty(2 * n * batch, dtype=dt, device=dev)
     pkg.fill_uniform(x.data_ptr(), x.numel(), 11, 0, prec)
     monkeypatch.setenv("NFFT_PERSIST", "0")
+    if n2:
+        monkeypatch.setenv("NFFT_FOUR_N2", str(n2))
     outs = {}
     for pf in ("1", "0"):
         monkeypatch.setenv("NFFT_ROWT_PF", pf)
         plan = pkg.Plan("dft", prec, [n], batch)
         assert "four-step" in plan.describe(), plan.describe()
+        if n2:
+            assert f"x{n2}" in plan.describe(), plan.describe()
         for op in ("dft", "invdft"):
             y = torch.empty_like(x)
             plan.execute(op, x.data_ptr(), y.data_ptr())
