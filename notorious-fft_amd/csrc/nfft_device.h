@@ -487,6 +487,11 @@ template <typename C> __device__ __forceinline__ C buf_ld(__amdgpu_buffer_rsrc_t
     else
         return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
 }
+// 16-byte stores take the whole offset in the lane register (scalar offset
+// 0): the compiler's wait state between a store of more than 8 bytes and a
+// following write of its data registers is only inserted for that form, and
+// FP64 results written through the scalar-offset form were intermittently
+// corrupted when other kernels ran at the same time.
 template <typename C>
 __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, C v)
 {
@@ -495,7 +500,7 @@ __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t voff, 
                                               (int)voff, (int)soff, 0);
     else
         __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v),
-                                               r, (int)voff, (int)soff, 0);
+                                               r, (int)(voff + soff), 0, 0);
 }
 
 // Logical workgroup index for XCD-aware tiling: workgroups are dispatched

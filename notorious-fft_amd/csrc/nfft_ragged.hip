@@ -92,7 +92,7 @@ template <int L> constexpr int rg_lines() { return RG_THREADS / Sched<L>::NT; }
 // groups of short lines
 // ---------------------------------------------------------------------------
 template <typename T, int L, int DIR, bool BLUE>
-__device__ void rg_line(const RagArgs &a, const RagItem &it, Cx<T> *lds, long long *bases)
+__device__ __noinline__ void rg_line(const RagArgs &a, const RagItem &it, Cx<T> *lds, long long *bases)
 {
     using S = Sched<L>;
     using C = Cx<T>;
@@ -303,7 +303,7 @@ __device__ void rg_rows(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, Cx<T> *l
 
 // one tile of a long power-of-two transform (four-step N1 x N2)
 template <typename T, int N1, int N2, int DIR>
-__device__ void rg_four(const RagArgs &a, const RagItem &it, Cx<T> *lds)
+__device__ __noinline__ void rg_four(const RagArgs &a, const RagItem &it, Cx<T> *lds)
 {
     using C = Cx<T>;
     const RagTab tb = a.tabs[it.tab];
@@ -315,7 +315,7 @@ __device__ void rg_four(const RagArgs &a, const RagItem &it, Cx<T> *lds)
 
 // one tile of a long Bluestein transform (three passes over M1 x M2)
 template <typename T, int M1, int M2, int DIR>
-__device__ void rg_blue3(const RagArgs &a, const RagItem &it, Cx<T> *lds)
+__device__ __noinline__ void rg_blue3(const RagArgs &a, const RagItem &it, Cx<T> *lds)
 {
     using C = Cx<T>;
     const RagTab tb = a.tabs[it.tab];
@@ -333,7 +333,7 @@ __device__ void rg_blue3(const RagArgs &a, const RagItem &it, Cx<T> *lds)
 
 // a group of short mixed-radix transforms (same length), LDS-resident
 template <typename T, int DIR>
-__device__ void rg_smooth_group(const RagArgs &a, const RagItem &it, Cx<T> *lds, long long *bases)
+__device__ __noinline__ void rg_smooth_group(const RagArgs &a, const RagItem &it, Cx<T> *lds, long long *bases)
 {
     using C = Cx<T>;
     const RagTab &tb = a.tabs[it.tab];
@@ -357,7 +357,7 @@ __device__ void rg_smooth_group(const RagArgs &a, const RagItem &it, Cx<T> *lds,
 
 // one tile of a long mixed-radix transform (four-step N1 x N2)
 template <typename T, int DIR>
-__device__ void rg_smooth4(const RagArgs &a, const RagItem &it, Cx<T> *lds)
+__device__ __noinline__ void rg_smooth4(const RagArgs &a, const RagItem &it, Cx<T> *lds)
 {
     using C = Cx<T>;
     const RagTab &tb = a.tabs[it.tab];
@@ -383,8 +383,12 @@ __device__ void rg_smooth4(const RagArgs &a, const RagItem &it, Cx<T> *lds)
 // ---------------------------------------------------------------------------
 // persistent kernel
 // ---------------------------------------------------------------------------
+// The item bodies are separate (not inlined) functions: inlined into one
+// switch they shared a register allocation of 395 (FP32) / 512 (FP64) VGPRs,
+// which left a single workgroup per CU.  Called, each keeps its own, and the
+// kernel fits 4 (FP32) / 2 (FP64) workgroups per CU.
 template <typename T, int DIR>
-__global__ void __launch_bounds__(RG_THREADS) k_ragged(RagArgs a)
+__global__ void __launch_bounds__(RG_THREADS, (sizeof(T) == 4 ? 4 : 2)) k_ragged(RagArgs a)
 {
     extern __shared__ char smem[];
     Cx<T> *lds = (Cx<T> *)smem;
@@ -796,7 +800,14 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
     if (B > 0) {
         const long long budget = (long long)std::max(16, env_int_rg("NFFT_RAGGED_MB", 512)) << 20;
         slots = std::max(1LL, std::min(B, budget / (slot_elems * (long long)cb)));
-        lag = std::min<long long>(std::max(0, env_int_rg("NFFT_RAGGED_LAG", 16)), (slots - 1) / 2);
+        // far enough that a later phase seldom finds the earlier one still
+        // running, short enough that the intermediates in flight (2 D
+        // transforms) stay near 96 MiB, inside the memory-side cache
+        double avg = 0;
+        for (const Long &l : longs) avg += (double)l.elems * (double)cb;
+        avg /= (double)B;
+        const int dflt = (int)std::max(16.0, std::min(1024.0, 96.0 * 1048576.0 / (2.0 * avg)));
+        lag = std::min<long long>(std::max(0, env_int_rg("NFFT_RAGGED_LAG", dflt)), (slots - 1) / 2);
     }
     std::vector<RagItem> sorted;
     std::vector<int> ids;
@@ -926,9 +937,10 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
             (void)hipEventCreate(&e1);
             float best = 1e30f;
             int best_pc = per_cu;
-            const int cands[] = {1, 2, 3, 4, 6, 8};
+            // at most what the kernel's register budget keeps resident
+            const int cands[] = {1, 2, 3, 4};
             for (int pc : cands) {
-                if (prec == F64 && pc > 4) continue;
+                if (prec == F64 && pc > 2) continue;
                 r->grid = std::max(1, std::min(r->nitems, ncu * pc));
                 (void)launch_ragged(r, 0, tin, tout, r->cap);
                 (void)hipEventRecord(e0, r->cap);

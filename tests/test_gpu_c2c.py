@@ -344,3 +344,39 @@ def test_rowT_prefetch_grid_bitwise(pkg, dev, prec, lg, n2, lgtot, monkeypatch):
         plan.close()
     for op in ("dft", "invdft"):
         assert torch.equal(outs[("1", op)], outs[("0", op)]), (op, (outs[("1", op)] - outs[("0", op)]).abs().max())
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_plans_created_while_others_execute(pkg, dev, prec):
+    """Plans built (tables and Bluestein filters on the setup stream) while
+    earlier plans' transforms still run on the caller's stream, with no
+    synchronisation in between; every transform against pocketfft.  FP64
+    stores of 16-byte elements through a scalar offset were once corrupted
+    intermittently in exactly this situation."""
+    import torch
+    from ragged_sizes import ragged_sizes
+    sizes = ragged_sizes(200, seed=2)
+    total = sum(sizes)
+    dt = torch.float32 if prec == "f32" else torch.float64
+    xd = torch.empty(2 * total, dtype=dt, device=dev)
+    pkg.fill_uniform(xd.data_ptr(), xd.numel(), 3, 0, prec)
+    x = xd.cpu().numpy().astype(np.float64)
+    for rnd in range(3):
+        yd = torch.empty_like(xd)
+        plans, off = {}, 0
+        for n in sizes:
+            if n not in plans:
+                plans[n] = pkg.Plan("dft", prec, [n], 1)
+            plans[n].execute("dft", xd[2 * off:].data_ptr(), yd[2 * off:].data_ptr())
+            off += n
+        torch.cuda.synchronize()
+        y = yd.cpu().numpy().astype(np.float64)
+        off = 0
+        for n in sizes:
+            want = np.fft.fft(x[2 * off:2 * (off + n):2] + 1j * x[2 * off + 1:2 * (off + n):2])
+            got = y[2 * off:2 * (off + n):2] + 1j * y[2 * off + 1:2 * (off + n):2]
+            err = ol.rel_l2(got, want)
+            assert err < tol_for(prec, n), (rnd, n, err, plans[n].describe())
+            off += n
+        for p in plans.values():
+            p.close()
