@@ -380,3 +380,4 @@ def test_plans_created_while_others_execute(pkg, dev, prec):
             off += n
         for p in plans.values():
             p.close()
+
