@@ -380,4 +380,3 @@ def test_plans_created_while_others_execute(pkg, dev, prec):
             off += n
         for p in plans.values():
             p.close()
-
