@@ -186,7 +186,7 @@ def config_extras(pkg, torch, only=None):
     plan_run("r2c2d_f32_4096^2", "realdft", "f32", [4096, 4096], 1, "realdft", p2, 4096 * 2049 * 2,
              2.5 * p2 * 24)
     n = 1 << 14
-    for op in ("dct2", "dst2", "dct4"):
+    for op in ("dct2", "dst2", "dct3", "dct4"):
         plan_run(f"{op}_f32_N2^14_x8192", "t2t3" if op != "dct4" else "t4", "f32", [n], 8192, op, n, n,
                  2.5 * n * math.log2(n))
     p2 = 2048 * 2048

@@ -155,8 +155,14 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs
     const long long rem = a.nlines - line0;
     const int nrows = rem < LPB ? (int)rem : LPB;
     const C *src = (const C *)((const T *)a.in + line0 * a.in_dist);
-    stage_in_rows<T, L, LPB, NTH>(lds, src, a.in_dist / 2, L, nrows);
-    __syncthreads();
+    // lines of >= 64 threads (one or more whole waves per line) read their
+    // permuted input straight from global memory, and DCT2/DST2 write their
+    // output straight back; shorter lines stage the line through LDS
+    constexpr bool DIRECT = S::NT >= 64;
+    if constexpr (!DIRECT) {
+        stage_in_rows<T, L, LPB, NTH>(lds, src, a.in_dist / 2, L, nrows);
+        __syncthreads();
+    }
     const C *line = lds + b * lpadlen<T, L>();
     C reg[S::R];
     LayLine<T, L> lay{b * lpadlen<T, L>()};
@@ -164,15 +170,37 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs
     if constexpr (OP == T_DCT2 || OP == T_DST2) {
         constexpr T sg = OP == T_DCT2 ? (T)1 : (T)-1;
         // v[i] = x[2i] (i < L), v[L+i] = sg * x[N-1-2i]; z[m] = (v[2m], v[2m+1])
-        auto xv = [&](int i) -> T {
-            return i < L ? lds_real<T, L>(line, 2 * i) : sg * lds_real<T, L>(line, N - 1 - 2 * (i - L));
-        };
+        if constexpr (DIRECT) {
+            // z[m] = (x[4m], x[4m+2]) for m < L/2, else (sg x[4c+3], sg x[4c+1])
+            // with c = L-1-m: two complex-pair loads of the real line each
+            const C *xl = (const C *)((const T *)a.in + (line0 + b) * a.in_dist);
 #pragma unroll
-        for (int q = 0; q < S::R; q++) {
-            const int m = j + S::NT * q;
-            reg[q] = cmk<C>(xv(2 * m), xv(2 * m + 1));
+            for (int q = 0; q < S::R; q++) {
+                const int m = j + S::NT * q;
+                C v = czero<T>();
+                if (b < nrows) {
+                    if (q < S::R / 2) {
+                        const C p0 = xl[2 * m], p1 = xl[2 * m + 1];
+                        v = cmk<C>(p0.x, p1.x);
+                    } else {
+                        const int c = L - 1 - m;
+                        const C p0 = xl[2 * c], p1 = xl[2 * c + 1];
+                        v = cmk<C>(sg * p1.y, sg * p0.y);
+                    }
+                }
+                reg[q] = v;
+            }
+        } else {
+            auto xv = [&](int i) -> T {
+                return i < L ? lds_real<T, L>(line, 2 * i) : sg * lds_real<T, L>(line, N - 1 - 2 * (i - L));
+            };
+#pragma unroll
+            for (int q = 0; q < S::R; q++) {
+                const int m = j + S::NT * q;
+                reg[q] = cmk<C>(xv(2 * m), xv(2 * m + 1));
+            }
+            __syncthreads();
         }
-        __syncthreads();
         line_fft<T, L, -1>(reg, lds, lay, j, (const double2 *)a.tw);
         C *ln = lds + b * lpadlen<T, L>();
 #pragma unroll
@@ -199,9 +227,34 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs
                 yb[q] = (T)-2 * p.y;
             }
         }
+        const T sc = (T)a.scale;
+        if constexpr (DIRECT) {
+            // y[k] and y[N-k] (DCT2) / y[k-1] and y[N-k-1] (DST2): each store
+            // instruction of a wave writes one contiguous run
+            if (b < nrows) {
+                T *y = (T *)a.out + (line0 + b) * a.out_dist;
+#pragma unroll
+                for (int q = 0; q < S::R; q++) {
+                    const int k = j + S::NT * q;
+                    int ia, ib;
+                    if constexpr (OP == T_DCT2) {
+                        ia = k;
+                        ib = N - k;
+                    } else {
+                        ia = k == 0 ? N - 1 : k - 1;
+                        ib = N - k - 1;
+                    }
+                    const T va = OP == T_DCT2 ? ya[q] : (k == 0 ? ya[q] : yb[q]);
+                    const T vb = OP == T_DCT2 ? yb[q] : ya[q];
+                    y[ia] = va * sc;
+                    if (k != 0) y[ib] = vb * sc;
+                    else y[OP == T_DCT2 ? L : L - 1] = yl * sc;
+                }
+            }
+            return;
+        }
         __syncthreads();
         // scatter y into the LDS line (real view), then a coalesced store
-        const T sc = (T)a.scale;
 #pragma unroll
         for (int q = 0; q < S::R; q++) {
             const int k = j + S::NT * q;
@@ -232,22 +285,28 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs
         // DST3: X[0] = w0 x[N-1], X[k] = conj((x[N-1-k] + i x[k-1]) w_k), X[L] = sqrt2 x[L-1]
         const C *dw = (const C *)a.t1;
         const C *rw = (const C *)a.t2;
+        // rows past nrows read row 0 (always present); tile_store drops them
+        const T *xg = (const T *)a.in + (line0 + (b < nrows ? b : 0)) * a.in_dist;
+        auto rx = [&](int i) -> T {
+            if constexpr (DIRECT) return xg[i];
+            else return lds_real<T, L>(line, i);
+        };
         auto X = [&](int k) -> C {
             if (k == 0) {
-                const T v = OP == T_DCT3 ? lds_real<T, L>(line, 0) : lds_real<T, L>(line, N - 1);
+                const T v = OP == T_DCT3 ? rx(0) : rx(N - 1);
                 return cmk<C>(v * (T)a.w0, (T)0);
             }
             if (k == L) {
-                const T v = OP == T_DCT3 ? lds_real<T, L>(line, L) : lds_real<T, L>(line, L - 1);
+                const T v = OP == T_DCT3 ? rx(L) : rx(L - 1);
                 return cmk<C>(sq2 * v, (T)0);
             }
             T xr, xi;
             if constexpr (OP == T_DCT3) {
-                xr = lds_real<T, L>(line, k);
-                xi = lds_real<T, L>(line, N - k);
+                xr = rx(k);
+                xi = rx(N - k);
             } else {
-                xr = lds_real<T, L>(line, N - 1 - k);
-                xi = lds_real<T, L>(line, k - 1);
+                xr = rx(N - 1 - k);
+                xi = rx(k - 1);
             }
             return cconj(cmul(cmk<C>(xr, xi), dw[k]));
         };
@@ -256,7 +315,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs
             const int k = j + S::NT * q;
             reg[q] = pack_pair(X(k), X(L - k), rw[k]);
         }
-        __syncthreads();
+        if constexpr (!DIRECT) __syncthreads();
         line_fft<T, L, 1>(reg, lds, lay, j, (const double2 *)a.tw);
         C *ln = lds + b * lpadlen<T, L>();
 #pragma unroll
@@ -291,20 +350,36 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t4_line(RtArgs 
     const long long rem = a.nlines - line0;
     const int nrows = rem < LPB ? (int)rem : LPB;
     const C *src = (const C *)((const T *)a.in + line0 * a.in_dist);
-    stage_in_rows<T, L, LPB, NTH>(lds, src, a.in_dist / 2, L, nrows);
-    __syncthreads();
-    const C *line = lds + b * lpadlen<T, L>();
     const C *pre = (const C *)a.t1;
     const C *post = (const C *)a.t2;
     C reg[S::R];
+    if constexpr (S::NT >= 64) {
+        // x[2n] = pair n .x, x[N-1-2n] = pair L-1-n .y, straight from global memory
+        const C *xl = src + (long long)b * (a.in_dist / 2);
 #pragma unroll
-    for (int q = 0; q < S::R; q++) {
-        const int n = j + S::NT * q;
-        const T xe = lds_real<T, L>(line, 2 * n), xo = lds_real<T, L>(line, N - 1 - 2 * n);
-        if constexpr (OP == T_DCT4) reg[q] = cmul(pre[n], cmk<C>(xe, xo));
-        else reg[q] = cneg(cmul(pre[n], cmk<C>(xe, -xo)));
+        for (int q = 0; q < S::R; q++) {
+            const int n = j + S::NT * q;
+            T xe = 0, xo = 0;
+            if (b < nrows) {
+                xe = xl[n].x;
+                xo = xl[L - 1 - n].y;
+            }
+            if constexpr (OP == T_DCT4) reg[q] = cmul(pre[n], cmk<C>(xe, xo));
+            else reg[q] = cneg(cmul(pre[n], cmk<C>(xe, -xo)));
+        }
+    } else {
+        stage_in_rows<T, L, LPB, NTH>(lds, src, a.in_dist / 2, L, nrows);
+        __syncthreads();
+        const C *line = lds + b * lpadlen<T, L>();
+#pragma unroll
+        for (int q = 0; q < S::R; q++) {
+            const int n = j + S::NT * q;
+            const T xe = lds_real<T, L>(line, 2 * n), xo = lds_real<T, L>(line, N - 1 - 2 * n);
+            if constexpr (OP == T_DCT4) reg[q] = cmul(pre[n], cmk<C>(xe, xo));
+            else reg[q] = cneg(cmul(pre[n], cmk<C>(xe, -xo)));
+        }
+        __syncthreads();
     }
-    __syncthreads();
     LayLine<T, L> lay{b * lpadlen<T, L>()};
     line_fft<T, L, -1>(reg, lds, lay, j, (const double2 *)a.tw);
     C *ln = lds + b * lpadlen<T, L>();
