@@ -91,14 +91,17 @@ template <int P, int DIR, typename C> __device__ __forceinline__ void dft_odd(C 
     v[0] = y0;
 }
 
-// n / d for 0 <= n < 2^24 and d >= 1, via the FP32 reciprocal and one
-// correction step (the estimate is within one of the quotient there).
-__device__ __forceinline__ int fdiv(int n, int d, float inv)
+// n / d for 0 <= n < 2^19 and 1 <= d < 2^24 from inv = rcp_up(d): the FP32
+// reciprocal raised by 2^-20 (relative) puts the FP32 product n * inv in
+// [n/d, n/d + n 2^-19 / d), below the next integer (at least 1/d above
+// n/d), so truncation gives the quotient with no correction step.
+__device__ __forceinline__ float rcp_up(int d)
 {
-    int q = (int)((float)n * inv);
-    const int r = n - q * d;
-    q += (r >= d) - (r < 0);
-    return q;
+    return (1.0f / (float)d) * 1.00000095367431640625f;
+}
+__device__ __forceinline__ int fdiv(int n, float inv)
+{
+    return (int)((float)n * inv);
 }
 
 template <int R, int DIR, typename C> __device__ __forceinline__ void dft_any(C *v)
@@ -109,29 +112,43 @@ template <int R, int DIR, typename C> __device__ __forceinline__ void dft_any(C 
 
 // One Stockham stage of radix R over G lines of length L (line g at
 // src/dst + g * L), all threads of the workgroup sharing the G*L/R butterflies.
-template <typename T, int R, int DIR>
+// TW: Ns > 1 (the first stage, Ns = 1, has no twiddles).
+template <typename T, int R, int DIR, bool TW>
 __device__ __forceinline__ void smooth_stage(const Cx<T> *src, Cx<T> *dst, int G, int L, int Ns, const Cx<T> *tw)
 {
     using C = Cx<T>;
     const int nb = L / R;
-    const int tws = L / (Ns * R);  // w_{Ns R}^{t k} = tw[t k tws], t k tws < L
-    const float inb = 1.0f / (float)nb, ins = 1.0f / (float)Ns;
+    const int tws = TW ? L / (Ns * R) : 0;  // w_{Ns R}^{t k} = tw[t k tws], t k tws < L
+    const float inb = rcp_up(nb), ins = rcp_up(Ns);
     for (int idx = threadIdx.x; idx < G * nb; idx += blockDim.x) {
-        const int g = fdiv(idx, nb, inb), j = idx - g * nb;
-        const int jq = fdiv(j, Ns, ins), k = j - jq * Ns;
-        const C *x = src + g * L + j;
+        const int g = fdiv(idx, inb), j = idx - __mul24(g, nb);
+        int jq = j, k = 0;
+        if constexpr (TW) {
+            jq = fdiv(j, ins);
+            k = j - __mul24(jq, Ns);
+        }
+        const C *x = src + __mul24(g, L) + j;
         C v[R];
 #pragma unroll
         for (int t = 0; t < R; t++) v[t] = x[t * nb];
-        if (Ns > 1) {
+        if constexpr (TW) {
+            const unsigned kt = (unsigned)__mul24(k, tws);
 #pragma unroll
-            for (int t = 1; t < R; t++) v[t] = ctw<DIR>(v[t], tw[t * k * tws]);
+            for (int t = 1; t < R; t++) v[t] = ctw<DIR>(v[t], tw[t * kt]);
         }
         dft_any<R, DIR>(v);
-        C *y = dst + g * L + jq * Ns * R + k;
+        C *y = dst + __mul24(g, L) + __mul24(jq, Ns * R) + k;
 #pragma unroll
         for (int t = 0; t < R; t++) y[t * Ns] = v[t];
     }
+}
+
+template <typename T, int R, int DIR>
+__device__ __forceinline__ void smooth_stage_any(const Cx<T> *src, Cx<T> *dst, int G, int L, int Ns,
+                                                 const Cx<T> *tw)
+{
+    if (Ns == 1) smooth_stage<T, R, DIR, false>(src, dst, G, L, 1, tw);
+    else smooth_stage<T, R, DIR, true>(src, dst, G, L, Ns, tw);
 }
 
 // All stages of G lines held in LDS buffer `a` (buffer `b` is the other half
@@ -145,12 +162,12 @@ __device__ Cx<T> *smooth_lines(Cx<T> *a, Cx<T> *b, int G, const SmoothPlan &sp, 
     for (int s = 0; s < sp.ns; s++) {
         const int r = sp.radix[s];
         switch (r) {
-        case 2: smooth_stage<T, 2, DIR>(src, dst, G, sp.L, Ns, tw); break;
-        case 3: smooth_stage<T, 3, DIR>(src, dst, G, sp.L, Ns, tw); break;
-        case 4: smooth_stage<T, 4, DIR>(src, dst, G, sp.L, Ns, tw); break;
-        case 5: smooth_stage<T, 5, DIR>(src, dst, G, sp.L, Ns, tw); break;
-        case 7: smooth_stage<T, 7, DIR>(src, dst, G, sp.L, Ns, tw); break;
-        default: smooth_stage<T, 8, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 2: smooth_stage_any<T, 2, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 3: smooth_stage_any<T, 3, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 4: smooth_stage_any<T, 4, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 5: smooth_stage_any<T, 5, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 7: smooth_stage_any<T, 7, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        default: smooth_stage_any<T, 8, DIR>(src, dst, G, sp.L, Ns, tw); break;
         }
         __syncthreads();
         Cx<T> *t = src;
@@ -186,9 +203,9 @@ template <typename T, int DIR> __global__ void __launch_bounds__(256) k_smooth_l
     const long long l0 = (long long)blockIdx.x * G;
     const int g_here = (int)((a.nlines - l0) < G ? (a.nlines - l0) : G);
     const C *src = (const C *)a.in;
-    const float iL = 1.0f / (float)L;
+    const float iL = rcp_up(L);
     for (int idx = threadIdx.x; idx < G * L; idx += blockDim.x) {
-        const int g = fdiv(idx, L, iL), p = idx - g * L;
+        const int g = fdiv(idx, iL), p = idx - __mul24(g, L);
         A[idx] = g < g_here ? src[(l0 + g) * a.in_dist + p] : cmk<C>((T)0, (T)0);
     }
     __syncthreads();
@@ -196,7 +213,7 @@ template <typename T, int DIR> __global__ void __launch_bounds__(256) k_smooth_l
     C *dst = (C *)a.out;
     const T sc = (T)a.scale;
     for (int idx = threadIdx.x; idx < G * L; idx += blockDim.x) {
-        const int g = fdiv(idx, L, iL), p = idx - g * L;
+        const int g = fdiv(idx, iL), p = idx - __mul24(g, L);
         if (g < g_here) {
             C v = res[idx];
             if (sc != (T)1) v = cscale(v, sc);
@@ -225,15 +242,15 @@ __device__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, const SmoothPlan &s
 {
     using C = Cx<T>;
     const int N1 = sp1.L;
-    const float iw = 1.0f / (float)w;
+    const float iw = rcp_up(w);
     for (int idx = threadIdx.x; idx < w * N1; idx += blockDim.x) {
-        const int p = fdiv(idx, w, iw), b = idx - p * w;
+        const int p = fdiv(idx, iw), b = idx - __mul24(p, w);
         A[b * N1 + p] = in[(long long)p * N2 + c0 + b];
     }
     __syncthreads();
     const C *res = smooth_lines<T, DIR>(A, B, w, sp1, tw1);
     for (int idx = threadIdx.x; idx < w * N1; idx += blockDim.x) {
-        const int k1 = fdiv(idx, w, iw), b = idx - k1 * w;
+        const int k1 = fdiv(idx, iw), b = idx - __mul24(k1, w);
         const uint32_t e = (uint32_t)k1 * (uint32_t)(c0 + b);
         const C v = ctw64<DIR>(res[b * N1 + k1], tw2_get(tw2, e));
         C *d = scr + (long long)k1 * N2 + c0 + b;
@@ -256,9 +273,9 @@ __device__ void smooth_row_tile(const Cx<T> *scr, Cx<T> *out, const SmoothPlan &
     }
     __syncthreads();
     const C *res = smooth_lines<T, DIR>(A, B, w, sp2, tw2t);
-    const float iw = 1.0f / (float)w;
+    const float iw = rcp_up(w);
     for (int idx = threadIdx.x; idx < w * N2; idx += blockDim.x) {
-        const int k2 = fdiv(idx, w, iw), b = idx - k2 * w;
+        const int k2 = fdiv(idx, iw), b = idx - __mul24(k2, w);
         C v = res[b * N2 + k2];
         if (scale != (T)1) v = cscale(v, scale);
         out[(long long)k2 * N1 + r0 + b] = v;

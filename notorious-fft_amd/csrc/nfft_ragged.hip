@@ -341,15 +341,16 @@ __device__ __noinline__ void rg_smooth_group(const RagArgs &a, const RagItem &it
     if (threadIdx.x < G) bases[threadIdx.x] = a.offs[a.ids[it.first + threadIdx.x]];
     __syncthreads();
     C *A = lds, *B = lds + G * L;
+    const float iL = rcp_up(L);
     for (int idx = threadIdx.x; idx < G * L; idx += RG_THREADS) {
-        const int g = idx / L, p = idx - g * L;
+        const int g = fdiv(idx, iL), p = idx - __mul24(g, L);
         A[idx] = ((const C *)a.in)[bases[g] + p];
     }
     __syncthreads();
     const C *res = smooth_lines<T, DIR>(A, B, G, tb.sp1, (const C *)tb.tw_a);
     const T sc = DIR > 0 ? (T)1 / (T)L : (T)1;  // inverse: 1/N for non-powers of two
     for (int idx = threadIdx.x; idx < G * L; idx += RG_THREADS) {
-        const int g = idx / L, p = idx - g * L;
+        const int g = fdiv(idx, iL), p = idx - __mul24(g, L);
         ((C *)a.out)[bases[g] + p] = cscale(res[idx], sc);
     }
     __syncthreads();
