@@ -399,6 +399,13 @@ struct Tw2 {
     int sh;
     int mask;
 };
+// The same two-level table rounded to FP32 (float2 entries).
+struct Tw2f {
+    const void *hi;
+    const void *lo;
+    int sh;
+    int mask;
+};
 __device__ __forceinline__ double2 tw2_get(const Tw2 &t, uint32_t e)
 {
     double2 h = t.hi[e >> t.sh], l = t.lo[e & t.mask];

@@ -171,6 +171,8 @@ struct C2C {
     long long scratch_per_line = 0;  // elements of Cx per line (ALG_FOUR: n, ALG_BLUE3: m)
     int col_lpb = 0, row_lpb = 0;     // tile widths of the two passes (0: defaults)
     int row_pf = 1;                   // row pass on the resident prefetching grid
+    int row_twin = 0;                 // inter-pass twiddle applied by the row pass
+    Tw2f tw2f{};                      // its FP32 table
     SmoothPlan sp{};                  // ALG_SMOOTH: stage schedule; ALG_SMOOTH4: N1
     SmoothPlan sp2{};                 // ALG_SMOOTH4: N2
     int sgroup = 1;                   // ALG_SMOOTH: lines per workgroup; ALG_SMOOTH4: columns per tile

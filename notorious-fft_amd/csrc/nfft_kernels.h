@@ -304,14 +304,43 @@ struct RowTArgs {
     double scale;
     int lpb;                      // rows per workgroup (0: default for L)
     int pf;                       // full tiles through k_rowT_pf (resident, prefetching)
+    Tw2f twf;                     // TWIN kernels: inter-pass twiddle w_N^{r p} of row r, point p
 };
+
+// TWIN: the row pass applies the four-step's inter-pass twiddle to its input
+// (row r = k1 of the column FFT, point p = n2) instead of the column pass to
+// its output.  The column pass of a 1024 x 1024 FP32 four-step is the slower
+// of the two, and the prefetching row grid hides the twiddle work under its
+// loads.  Rows of 1024 FP32 points only: a thread's base and step exponents
+// r j and r 64 stay below N / 16, so the row grid keeps the used part of the
+// two-level table (hi[0, 64), lo[0, 2048)) in LDS, and no global load joins
+// the prefetch in the in-order memory counter.
+constexpr int kTwinHi = 64, kTwinLo = 2048;
+template <int DIR, int R>
+__device__ __forceinline__ void apply_twin(Cx<float> (&reg)[R], const Cx<float> *hi, const Cx<float> *lo, int sh,
+                                           int mask, uint32_t base_e, uint32_t step_e)
+{
+    const Cx<float> w = cmul(hi[base_e >> sh], lo[base_e & mask]);
+    const Cx<float> st = cmul(hi[step_e >> sh], lo[step_e & mask]);
+    Cx<float> e = w, o = cmul(w, st);
+    const Cx<float> st2 = cmul(st, st);
+#pragma unroll
+    for (int q = 0; q < R; q += 2) {
+        reg[q] = ctw<DIR>(reg[q], e);
+        reg[q + 1] = ctw<DIR>(reg[q + 1], o);
+        if (q + 2 < R) {
+            e = cmul(e, st2);
+            o = cmul(o, st2);
+        }
+    }
+}
 
 template <typename T, int L, int LPB> __host__ __device__ constexpr int rowT_lds()
 {
     return (L * (LPB + 1)) > line_lds<T, L, LPB>() ? L * (LPB + 1) : line_lds<T, L, LPB>();
 }
 
-template <typename T, int L, int LPB, int DIR>
+template <typename T, int L, int LPB, int DIR, bool TWIN = false>
 __global__ void __launch_bounds__(Sched<L>::NT *LPB,
                                   (wpe_hint<T>(Sched<L>::NT * LPB, (long)rowT_lds<T, L, LPB>() * sizeof(Cx<T>))))
 k_rowT(RowTArgs a)
@@ -330,6 +359,12 @@ k_rowT(RowTArgs a)
     const C *src = (const C *)a.in + t * a.in_dist + (long long)r0 * L;
     C reg[S::R];
     tile_load<T, L, LPB>(reg, lds, src, L, L, nrows, b, j);
+    if constexpr (TWIN) {
+        // the table read from global memory; the same arithmetic as k_rowT_pf
+        const uint32_t rr = b < nrows ? (uint32_t)(r0 + b) : 0u;
+        apply_twin<DIR>(reg, (const Cx<float> *)a.twf.hi, (const Cx<float> *)a.twf.lo, a.twf.sh, a.twf.mask,
+                        rr * (uint32_t)j, rr * (uint32_t)S::NT);
+    }
     LayLine<T, L> lay{b * lpadlen<T, L>()};
     line_fft<T, L, DIR>(reg, lds, lay, j, (const double2 *)a.tw);
     const T sc = (T)a.scale;
@@ -383,7 +418,7 @@ template <typename T, int L, int LPB> __host__ __device__ constexpr int rowT_pf_
     return rowT_lds<T, L, LPB>() + stage_tw_lds<T, L>();
 }
 
-template <typename T, int L, int LPB, int DIR>
+template <typename T, int L, int LPB, int DIR, bool TWIN = false>
 __global__ void __launch_bounds__(Sched<L>::NT *LPB,
                                   (wpe_hint<T>(Sched<L>::NT * LPB, (long)rowT_pf_lds<T, L, LPB>() * sizeof(Cx<T>))))
 k_rowT_pf(RowTArgs a, long long ntiles)
@@ -394,6 +429,7 @@ k_rowT_pf(RowTArgs a, long long ntiles)
     constexpr int KS = NTH / LPB;  // output points k advanced per sweep
     __shared__ C lds[rowT_lds<T, L, LPB>()];
     __shared__ C ltw[stage_tw_lds<T, L>()];
+    __shared__ C thi[TWIN ? kTwinHi : 1], tlo[TWIN ? kTwinLo : 1];
     const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
     const unsigned gpt = (unsigned)(a.nrows / LPB);
     const unsigned x = blockIdx.x & 7, m = gridDim.x >> 3;
@@ -407,6 +443,10 @@ k_rowT_pf(RowTArgs a, long long ntiles)
     unsigned tl = lo + (blockIdx.x >> 3);
     if (tl >= hi) return;  // uniform: no tile for this workgroup
     stage_tw_to_lds<T, L>(ltw, (const double2 *)a.tw);
+    if constexpr (TWIN) {
+        for (int i = threadIdx.x; i < kTwinHi; i += NTH) thi[i] = ((const C *)a.twf.hi)[i];
+        for (int i = threadIdx.x; i <= a.twf.mask; i += NTH) tlo[i] = ((const C *)a.twf.lo)[i];
+    }
     C reg[S::R], nxt[S::R];
     {
         const unsigned t = tl / gpt;
@@ -430,6 +470,10 @@ k_rowT_pf(RowTArgs a, long long ntiles)
                 buf_rsrc_if((const C *)a.in + u * a.in_dist + (tn - u * gpt) * LPB * (long long)L, tl + m < hi);
 #pragma unroll
             for (int q = 0; q < S::R; q++) nxt[q] = buf_ld<C>(rs, loff, q * S::NT * (uint32_t)sizeof(C));
+        }
+        if constexpr (TWIN) {
+            const uint32_t rr = (uint32_t)(r0 + b);
+            apply_twin<DIR>(reg, thi, tlo, a.twf.sh, a.twf.mask, rr * (uint32_t)j, rr * (uint32_t)S::NT);
         }
         LayLine<T, L> lay{b * lpadlen<T, L>()};
         line_fft_tw<T, L, DIR>(reg, lds, lay, j, StageTwLds<T>{ltw});

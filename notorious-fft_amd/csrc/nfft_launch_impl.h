@@ -145,14 +145,14 @@ template <typename T, int L, int LPB> constexpr bool rowT_pf_fits()
            (size_t)rowT_pf_lds<T, L, LPB>() * sizeof(Cx<T>) <= kLdsLimit;
 }
 
-template <typename T, int L, int LPB, int DIR> bool rowT_pf_try(const RowTArgs &a, hipStream_t s)
+template <typename T, int L, int LPB, int DIR, bool TWIN> bool rowT_pf_try(const RowTArgs &a, hipStream_t s)
 {
     if (a.nrows % LPB != 0 || (long long)L * a.nrows * (long long)sizeof(Cx<T>) >= (1LL << 32)) return false;
     static const long long resident = [] {
         int dev = 0, ncu = 0, n = 0;
         if (hipGetDevice(&dev) != hipSuccess ||
             hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_rowT_pf<T, L, LPB, DIR>,
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, (const void *)k_rowT_pf<T, L, LPB, DIR, TWIN>,
                                                          Sched<L>::NT * LPB, 0) != hipSuccess)
             return 0LL;
         return (long long)n * ncu;
@@ -161,23 +161,43 @@ template <typename T, int L, int LPB, int DIR> bool rowT_pf_try(const RowTArgs &
     if (ntiles >= (1LL << 31)) return false;  // 32-bit tile indices
     const long long g = std::min(resident, (ntiles + 7) / 8 * 8) / 8 * 8;
     if (g < 8) return false;
-    hipLaunchKernelGGL((k_rowT_pf<T, L, LPB, DIR>), dim3((unsigned)g), dim3(Sched<L>::NT * LPB), 0, s, a, ntiles);
+    hipLaunchKernelGGL((k_rowT_pf<T, L, LPB, DIR, TWIN>), dim3((unsigned)g), dim3(Sched<L>::NT * LPB), 0, s, a, ntiles);
     return true;
+}
+
+// Rows that take the inter-pass twiddle (RowTArgs::twf set): FP32 rows of
+// 1024 points, the row pass of the 2^14..2^21-point four-steps.
+template <typename T, int L> constexpr bool rowT_twin_v() { return sizeof(T) == 4 && L == 1024; }
+
+template <typename T, int L, int LPB, bool TWIN> constexpr bool rowT_pf_fits_t()
+{
+    return rowT_pf_fits<T, L, LPB>() &&
+           (!TWIN || (size_t)(rowT_pf_lds<T, L, LPB>() + kTwinHi + kTwinLo) * sizeof(Cx<T>) <= kLdsLimit);
+}
+
+template <typename T, int L, int LPB, bool TWIN> hipError_t rowT_launch_t(int dir, const RowTArgs &a, hipStream_t s)
+{
+    if (TWIN && a.twf.mask >= kTwinLo) return hipErrorInvalidValue;
+    if constexpr (rowT_pf_fits_t<T, L, LPB, TWIN>()) {
+        if (a.pf && (dir < 0 ? rowT_pf_try<T, L, LPB, -1, TWIN>(a, s) : rowT_pf_try<T, L, LPB, 1, TWIN>(a, s)))
+            return hipGetLastError();
+    }
+    unsigned gy, gz;
+    grid_trans(a.ntrans, gy, gz);
+    dim3 g((unsigned)((a.nrows + LPB - 1) / LPB), gy, gz), b(Sched<L>::NT * LPB);
+    if (dir < 0) hipLaunchKernelGGL((k_rowT<T, L, LPB, -1, TWIN>), g, b, 0, s, a);
+    else hipLaunchKernelGGL((k_rowT<T, L, LPB, 1, TWIN>), g, b, 0, s, a);
+    return hipGetLastError();
 }
 
 template <typename T, int L, int LPB> hipError_t rowT_launch(int dir, const RowTArgs &a, hipStream_t s)
 {
     if constexpr (rowT_fits_v<T, L, LPB>()) {
-        if constexpr (rowT_pf_fits<T, L, LPB>()) {
-            if (a.pf && (dir < 0 ? rowT_pf_try<T, L, LPB, -1>(a, s) : rowT_pf_try<T, L, LPB, 1>(a, s)))
-                return hipGetLastError();
+        if (a.twf.hi) {
+            if constexpr (rowT_twin_v<T, L>()) return rowT_launch_t<T, L, LPB, true>(dir, a, s);
+            return hipErrorInvalidValue;
         }
-        unsigned gy, gz;
-        grid_trans(a.ntrans, gy, gz);
-        dim3 g((unsigned)((a.nrows + LPB - 1) / LPB), gy, gz), b(Sched<L>::NT * LPB);
-        if (dir < 0) hipLaunchKernelGGL((k_rowT<T, L, LPB, -1>), g, b, 0, s, a);
-        else hipLaunchKernelGGL((k_rowT<T, L, LPB, 1>), g, b, 0, s, a);
-        return hipGetLastError();
+        return rowT_launch_t<T, L, LPB, false>(dir, a, s);
     }
     return hipErrorInvalidValue;
 }

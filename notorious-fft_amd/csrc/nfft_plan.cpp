@@ -190,6 +190,21 @@ const void *phase_table(int prec, long long count, long long a, long long c, lon
     });
 }
 
+// FP32 copy of the two-level table; rows of 1024 points read at most the
+// first kTwinHi hi entries (exponents below N / 16).
+bool tw2f_table(long long n, Tw2f *out)
+{
+    const int lg = ilog2ll(n);
+    const int sh = (lg + 1) / 2;
+    const long long lo_n = 1LL << sh, hi_n = (n + lo_n - 1) >> sh;
+    if (lo_n > kTwinLo || hi_n < kTwinHi || ((n / 1024 - 1) * 64 >> sh) >= kTwinHi) return false;
+    out->hi = phase_table(F32, hi_n, lo_n, 0, n);
+    out->lo = phase_table(F32, lo_n, 1, 0, n);
+    out->sh = sh;
+    out->mask = (int)(lo_n - 1);
+    return out->hi && out->lo;
+}
+
 bool tw2_table(long long n, Tw2 *out)
 {
     const int lg = ilog2ll(n);
@@ -406,6 +421,10 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
             col_lpb = env_int("NFFT_COL_LPB", 0);
             row_lpb = env_int("NFFT_ROWT_LPB", 0);
             row_pf = env_int("NFFT_ROWT_PF", 1) != 0 ? 1 : 0;
+            // FP32 1024-point rows take the inter-pass twiddle (column pass
+            // 992 -> 910 us per 2^28 points; the prefetching row grid absorbs it)
+            row_twin = prec == F32 && n2 == 1024 && env_int("NFFT_ROW_TWIDDLE", 1) != 0 ? 1 : 0;
+            if (row_twin && !tw2f_table(n, &tw2f)) row_twin = 0;
         }
         tw_a = stage_table(prec, n1);
         tw_b = stage_table(prec, n2);
@@ -630,10 +649,11 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
             c.tw2 = tw2;
             c.scale = 1.0;
             c.lpb = col_lpb;
-            hipError_t e = launch_col(prec, n1, dir, COL_TWIDDLE, c, s);
+            hipError_t e = launch_col(prec, n1, dir, row_twin ? COL_PLAIN : COL_TWIDDLE, c, s);
             if (e != hipSuccess) return e;
             RowTArgs r{scratch, (char *)out + (size_t)l0 * out_dist * cb, n1, n, out_dist, cnt, tw_b, scale, row_lpb,
                        row_pf};
+            if (row_twin) r.twf = tw2f;
             e = launch_rowT(prec, n2, dir, r, s);
             if (e != hipSuccess) return e;
         }

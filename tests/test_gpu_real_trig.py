@@ -35,8 +35,8 @@ def test_realdft(pkg, dev, prec, n):
 
 
 TRIG = ["dct2", "dst2", "dct3", "dst3", "dct4", "dst4"]
-T32 = [1, 2, 4, 8, 16, 128, 1024, 16384, 32768, 1 << 16]
-T64 = [1, 2, 4, 32, 1024, 16384, 1 << 15]
+T32 = [1, 2, 4, 8, 16, 128, 1024, 2048, 4096, 16384, 32768, 1 << 16]
+T64 = [1, 2, 4, 32, 1024, 2048, 16384, 1 << 15]
 
 
 @pytest.mark.parametrize("op", TRIG)
