@@ -593,7 +593,12 @@ __global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT_r2c(RowR2CArgs a)
     using C = Cx<T>;
     static_assert(LPB % 2 == 0, "row pairs");
     constexpr int NTH = S::NT * LPB, PADL = lpadlen<T, L>();
+    // unpack twiddle w[k2 n1 + r] = w[k2 n1] w[r]: the L column factors and
+    // the LPB row factors of this workgroup in LDS (where they fit beside the
+    // tile) instead of one global load per output point
+    constexpr bool WLDS = (size_t)(LPB * PADL + L + LPB) * sizeof(C) <= 160 * 1024;
     __shared__ C lds[LPB * PADL];
+    __shared__ C wtab[WLDS ? L + LPB : 1];
     const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
     const long long lin = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
     const long long lg = xcd_logical(lin, (long long)gridDim.x * gridDim.y * gridDim.z);
@@ -608,6 +613,14 @@ __global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT_r2c(RowR2CArgs a)
         return (slot & 1) ? (p == 0 ? npairs : n1 - p) : p;
     };
     const C *src = (const C *)a.in + t * a.in_dist;
+    const C *w = (const C *)a.w;
+    if constexpr (WLDS) {
+        for (int i = threadIdx.x; i < L; i += NTH) wtab[i] = w[(long long)i * n1];
+        if (threadIdx.x < LPB) {
+            const int r = row_of(threadIdx.x);
+            wtab[L + threadIdx.x] = w[r >= 0 ? r : 0];
+        }
+    }
     C reg[S::R];
     if constexpr (S::NT >= 64) {
         const int r = row_of(b);
@@ -631,7 +644,6 @@ __global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT_r2c(RowR2CArgs a)
     __syncthreads();
     const long long Lc = (long long)n1 * L;
     C *dst = (C *)a.out + t * a.out_dist;
-    const C *w = (const C *)a.w;
     const T sc = (T)a.scale, sk = a.dbl ? (T)2 * sc : sc;
     for (int idx = threadIdx.x; idx < LPB * L; idx += NTH) {
         const int slot = idx % LPB, k2 = idx / LPB;
@@ -646,8 +658,11 @@ __global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT_r2c(RowR2CArgs a)
         }
         const C zk = lds[slot * PADL + lpad<T>(k2)], zm = lds[ps * PADL + lpad<T>(pc)];
         const long long k = (long long)k2 * n1 + r;
+        C wk;
+        if constexpr (WLDS) wk = cmul(wtab[k2], wtab[L + slot]);
+        else wk = w[k];
         C xk, xl;
-        unpack_pair(zk, zm, w[k], xk, xl);
+        unpack_pair(zk, zm, wk, xk, xl);
         dst[k] = cscale(xk, k == 0 ? sc : sk);
         if (k == 0) dst[Lc] = cscale(xl, sc);
     }
