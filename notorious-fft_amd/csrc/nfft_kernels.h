@@ -211,7 +211,17 @@ k_col(ColArgs a)
     using S = Sched<L>;
     using C = Cx<T>;
     constexpr int NLDS = S::S > 1 ? L * LPB : 1;
+    // plain / twiddle columns read their stage twiddles from an LDS copy
+    // (rounded once to T) where it fits beside the tile without lowering the
+    // number of workgroups the LDS admits per CU (1 KiB allocation steps)
+    constexpr int TWN = Sched<L>::TWN > 0 ? Sched<L>::TWN : 1;
+    constexpr size_t kB0 = ((size_t)NLDS * sizeof(C) + 1023) / 1024 * 1024;
+    constexpr size_t kB1 = ((size_t)(NLDS + TWN) * sizeof(C) + 1023) / 1024 * 1024;
+    constexpr bool TWL = S::S > 1 && (MODE == COL_PLAIN || MODE == COL_TWIDDLE) && kB1 <= 160 * 1024 &&
+                         (160 * 1024) / kB1 == (160 * 1024) / kB0;
     __shared__ C lds[NLDS];
+    __shared__ C ltw[TWL ? TWN : 1];
+    if constexpr (TWL) stage_tw_to_lds<T, L>(ltw, (const double2 *)a.tw);
     const int b = threadIdx.x % LPB, j = threadIdx.x / LPB;
     const long long lin = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
     const long long lg = xcd_logical(lin, (long long)gridDim.x * gridDim.y * gridDim.z);
@@ -248,7 +258,8 @@ k_col(ColArgs a)
         for (int q = 0; q < S::R; q++) reg[q] = valid ? src[(long long)(j + S::NT * q) * a.S] : czero<T>();
     }
     LayCol<LPB> lay{b};
-    line_fft<T, L, DIR>(reg, lds, lay, j, (const double2 *)a.tw);
+    if constexpr (TWL) line_fft_tw<T, L, DIR>(reg, lds, lay, j, StageTwLds<T>{ltw});
+    else line_fft<T, L, DIR>(reg, lds, lay, j, (const double2 *)a.tw);
     if constexpr (MODE == COL_TWIDDLE || MODE == COL_BLUE_IN) {
         const uint32_t cc = valid ? (uint32_t)c : 0u;
         apply_tw2_series<DIR>(reg, a.tw2, cc * (uint32_t)j, cc * (uint32_t)S::NT);
