@@ -9,13 +9,14 @@ namespace nfft {
 namespace {
 template <typename T, int DIR> void smooth_attr()
 {
-    static bool done = false;  // one attribute call per instantiation and process
-    if (!done) {
+    // one attribute call per instantiation and process (thread-safe static init)
+    static const bool done = [] {
         (void)hipFuncSetAttribute((const void *)k_smooth_line<T, DIR>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
         (void)hipGetLastError();
-        done = true;
-    }
+        return true;
+    }();
+    (void)done;
 }
 
 template <typename T> hipError_t smooth_line_impl(int dir, const SmoothArgs &a, hipStream_t s)
@@ -35,15 +36,15 @@ template <typename T> hipError_t smooth_line_impl(int dir, const SmoothArgs &a, 
 }
 template <typename T, int DIR> void smooth4_attr()
 {
-    static bool done = false;
-    if (!done) {
+    static const bool done = [] {
         (void)hipFuncSetAttribute((const void *)k_smooth_col<T, DIR>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
         (void)hipFuncSetAttribute((const void *)k_smooth_rowT<T, DIR>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   160 * 1024);
         (void)hipGetLastError();
-        done = true;
-    }
+        return true;
+    }();
+    (void)done;
 }
 
 template <typename T> hipError_t smooth4_impl(int dir, const Smooth4Args &a, hipStream_t s)
