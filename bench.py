@@ -439,7 +439,7 @@ def main():
     if not args.no_sweep:
         for prec in ("f32", "f64"):
             for lg in range(10, 21):
-                r = run_r2c(pkg, torch, 1 << lg, prec, 1 << 28, steps=10, warmup=5)
+                r = run_r2c(pkg, torch, 1 << lg, prec, 1 << 28, steps=10, warmup=10)
                 sweep_r2c.append(r)
                 log(f"  sweep r2c {prec} N=2^{lg}: {r['gflops']:.1f} GFLOP/s {100 * r['hbm_frac']:.1f}% HBM "
                     f"rel-L2 {r['rel_l2']:.2e} [{r['plan']}]")
@@ -449,7 +449,7 @@ def main():
         for prec in ("f32", "f64"):
             for lg in range(10, 21):
                 n = 1 << lg
-                r = run_config(pkg, torch, n, prec, total, 10, 5, profile=True, check=1)
+                r = run_config(pkg, torch, n, prec, total, 10, 10, profile=True, check=1)
                 kern = r["kernels"]
                 dom = max(kern, key=lambda k: kern[k]["total_ms"])
                 sweep.append({"n": n, "prec": prec, "gflops": round(r["gflops"], 1),
