@@ -597,8 +597,12 @@ __global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT_r2c(RowR2CArgs a)
     // the LPB row factors of this workgroup in LDS (where they fit beside the
     // tile) instead of one global load per output point
     constexpr bool WLDS = (size_t)(LPB * PADL + L + LPB) * sizeof(C) <= 160 * 1024;
+    // stage twiddles from an LDS copy too where that still fits
+    constexpr int TWN = Sched<L>::TWN > 0 ? Sched<L>::TWN : 1;
+    constexpr bool TWL = S::S > 1 && WLDS && (size_t)(LPB * PADL + L + LPB + TWN) * sizeof(C) <= 160 * 1024;
     __shared__ C lds[LPB * PADL];
     __shared__ C wtab[WLDS ? L + LPB : 1];
+    __shared__ C ltw[TWL ? TWN : 1];
     const int b = threadIdx.x / S::NT, j = threadIdx.x % S::NT;
     const long long lin = blockIdx.x + (long long)gridDim.x * (blockIdx.y + (long long)gridDim.y * blockIdx.z);
     const long long lg = xcd_logical(lin, (long long)gridDim.x * gridDim.y * gridDim.z);
@@ -614,6 +618,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT_r2c(RowR2CArgs a)
     };
     const C *src = (const C *)a.in + t * a.in_dist;
     const C *w = (const C *)a.w;
+    if constexpr (TWL) stage_tw_to_lds<T, L>(ltw, (const double2 *)a.tw);
     if constexpr (WLDS) {
         for (int i = threadIdx.x; i < L; i += NTH) wtab[i] = w[(long long)i * n1];
         if (threadIdx.x < LPB) {
@@ -638,7 +643,8 @@ __global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT_r2c(RowR2CArgs a)
         __syncthreads();
     }
     LayLine<T, L> lay{b * PADL};
-    line_fft<T, L, -1>(reg, lds, lay, j, (const double2 *)a.tw);
+    if constexpr (TWL) line_fft_tw<T, L, -1>(reg, lds, lay, j, StageTwLds<T>{ltw});
+    else line_fft<T, L, -1>(reg, lds, lay, j, (const double2 *)a.tw);
 #pragma unroll
     for (int q = 0; q < S::R; q++) lds[b * PADL + lpad<T>(j + S::NT * q)] = reg[q];
     __syncthreads();
