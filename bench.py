@@ -404,7 +404,7 @@ def main():
     if only:
         args.no_sweep = args.no_cpu = True
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
               "batched 1D transforms shard by splitting the batch, with no exchange", file=sys.stderr)
         sys.exit(2)
 

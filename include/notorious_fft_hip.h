@@ -33,7 +33,7 @@ enum {
     NFFT_E_SIZE = 2,        /* size not supported (the reference's NULL-aux cases) */
     NFFT_E_ALLOC = 3,       /* device or host allocation failed */
     NFFT_E_HIP = 4,         /* a HIP runtime call or kernel launch failed */
-    NFFT_E_UNSUPPORTED = 5  /* valid request outside this build's limits */
+    NFFT_E_UNSUPPORTED = 5  /* valid request outside the library's limits */
 };
 
 /* precision */

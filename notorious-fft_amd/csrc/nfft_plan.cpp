@@ -415,7 +415,7 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
             n2 = 1 << lg2;
             n1 = n / n2;
             if (n1 > col_max(prec)) {
-                set_error("power-of-two length " + std::to_string(n) + " exceeds this build's limit");
+                set_error("power-of-two length " + std::to_string(n) + " exceeds the library's limit");
                 return false;
             }
             col_lpb = env_int("NFFT_COL_LPB", 0);
@@ -487,7 +487,7 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
         m2 = 1 << lg2;
         m1 = m / m2;
         if (m1 > col_max(prec)) {
-            set_error("Bluestein length " + std::to_string(n) + " exceeds this build's limit");
+            set_error("Bluestein length " + std::to_string(n) + " exceeds the library's limit");
             return false;
         }
         algo = ALG_BLUE3;

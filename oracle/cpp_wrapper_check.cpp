@@ -5,7 +5,7 @@
 // include/notorious_fft.h and linked with libnotorious_fft_hip_f64.so: the
 // drop-in claim for the wrapper's callers.  oracle/Makefile builds it into
 // oracle/_ref/ (the wrapper is read where it lies under the reference tree);
-// tests/test_gpu_cpp_wrapper.py runs it on the GPU box.
+// tests/test_gpu_cpp_wrapper.py runs it on the GPU.
 //
 // Each check compares a wrapper call with a long-double direct sum or with a
 // round trip of known scale (unnormalised transforms: invdft(dft(x)) = N x

@@ -1,7 +1,7 @@
 /*
  * nfft_oracle.c -- CPU restatement of the reference FFT algorithms (FP64).
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity reference: only tests/, the
  * smoke entry point and the CPU-baseline leg of bench.py may load it.  The
  * product (the HIP library) never links it and has no CPU fallback.
  *

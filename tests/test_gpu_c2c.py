@@ -2,7 +2,7 @@
 Bluestein) against the oracle, through the C ABI.
 
 Tolerances (rel-L2 against the FP64 oracle applied to the same FP32-rounded
-inputs): FP32 1e-6 * log2 N, FP64 1e-14 * log2 N.  Bluestein FP64 is judged
+inputs): FP32 1e-6 * log2 N, FP64 1e-14 * log2 N.  Bluestein FP64 is checked
 against numpy.fft (pocketfft), because the reference's own chirp phase is not
 reduced mod 2N and carries an error growing with N; the distance to the
 reference's algorithm is bounded by its own test tolerance (1e-9,
