@@ -44,9 +44,14 @@ def test_pow2_single_pass(pkg, dev, prec, n):
 
 
 @pytest.mark.parametrize("prec,n,batch", [("f32", 1 << 15, 3), ("f32", 1 << 16, 2), ("f32", 1 << 18, 2),
-                                          ("f32", 1 << 20, 2), ("f64", 1 << 14, 3), ("f64", 1 << 17, 2),
-                                          ("f64", 1 << 20, 1)])
-def test_pow2_four_step(pkg, dev, prec, n, batch):
+                                          ("f32", 1 << 20, 2), ("f32", 1 << 21, 1), ("f64", 1 << 14, 3),
+                                          ("f64", 1 << 17, 2), ("f64", 1 << 20, 1)])
+@pytest.mark.parametrize("row_twiddle", ["1", "0"])
+def test_pow2_four_step(pkg, dev, prec, n, batch, row_twiddle, monkeypatch):
+    """FP32 four-steps with 1024-point rows apply the inter-pass twiddle in the
+    row pass (NFFT_ROW_TWIDDLE=1, the default); 0 keeps it in the column pass
+    as every other four-step does.  Both against the oracle."""
+    monkeypatch.setenv("NFFT_ROW_TWIDDLE", row_twiddle)
     x = rounded(ol.uniform(2, 2 * n * batch), prec)
     for op in ("dft", "invdft"):
         got = run_plan(pkg, dev, op, [n], x, batch, prec)
