@@ -7,32 +7,49 @@
 namespace nfft {
 
 namespace {
-template <typename T, int DIR> void smooth_attr()
+template <typename T, int DIR, bool TWL> void smooth_attr()
 {
     // one attribute call per instantiation and process (thread-safe static init)
     static const bool done = [] {
-        (void)hipFuncSetAttribute((const void *)k_smooth_line<T, DIR>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024);
+        (void)hipFuncSetAttribute((const void *)k_smooth_line<T, DIR, TWL>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
         (void)hipGetLastError();
         return true;
     }();
     (void)done;
 }
 
-template <typename T> hipError_t smooth_line_impl(int dir, const SmoothArgs &a, hipStream_t s)
+template <typename T, bool TWL> hipError_t smooth_line_launch(int dir, const SmoothArgs &a, hipStream_t s)
 {
     const long long grid = (a.nlines + a.G - 1) / a.G;
-    const size_t lds = 2 * (size_t)a.G * a.sp.L * sizeof(Cx<T>);
+    const size_t lds = (2 * (size_t)a.G + (TWL ? 1 : 0)) * a.sp.L * sizeof(Cx<T>);
     if (grid < 1) return hipSuccess;
     if (grid > 0x7fffffffLL || lds > 160 * 1024) return hipErrorInvalidValue;
     if (dir < 0) {
-        smooth_attr<T, -1>();
-        hipLaunchKernelGGL((k_smooth_line<T, -1>), dim3((unsigned)grid), dim3(256), lds, s, a);
+        smooth_attr<T, -1, TWL>();
+        hipLaunchKernelGGL((k_smooth_line<T, -1, TWL>), dim3((unsigned)grid), dim3(256), lds, s, a);
     } else {
-        smooth_attr<T, 1>();
-        hipLaunchKernelGGL((k_smooth_line<T, 1>), dim3((unsigned)grid), dim3(256), lds, s, a);
+        smooth_attr<T, 1, TWL>();
+        hipLaunchKernelGGL((k_smooth_line<T, 1, TWL>), dim3((unsigned)grid), dim3(256), lds, s, a);
     }
     return hipGetLastError();
+}
+
+// FP32 lines read their twiddles from LDS when a workgroup can give up lines
+// for them within the same LDS size (2 G L >= 2 G' L + L with G' >= 1):
+// measured N = 100 0.815 -> 0.78 ms, N = 480 0.958 -> 0.879 ms per 2^27
+// points.  FP64 lines lost with it (N = 100 1.23 -> 1.31 ms) and keep the
+// global table.
+template <typename T> hipError_t smooth_line_impl(int dir, const SmoothArgs &a, hipStream_t s)
+{
+    const int L = a.sp.L;
+    const int g2 = (2 * a.G * L - L) / (2 * L);
+    if (sizeof(T) == 4 && g2 >= 1) {
+        SmoothArgs b = a;
+        b.G = g2;
+        if constexpr (sizeof(T) == 4) return smooth_line_launch<T, true>(dir, b, s);
+    }
+    return smooth_line_launch<T, false>(dir, a, s);
 }
 template <typename T, int DIR> void smooth4_attr()
 {

@@ -193,7 +193,9 @@ struct SmoothArgs {
     double scale;
 };
 
-template <typename T, int DIR> __global__ void __launch_bounds__(256) k_smooth_line(SmoothArgs a)
+// TWL: the L twiddles are copied into LDS after the two line buffers (the
+// launcher gives up lines per workgroup for them, keeping the LDS size).
+template <typename T, int DIR, bool TWL = false> __global__ void __launch_bounds__(256) k_smooth_line(SmoothArgs a)
 {
     using C = Cx<T>;
     extern __shared__ char smem[];
@@ -203,13 +205,19 @@ template <typename T, int DIR> __global__ void __launch_bounds__(256) k_smooth_l
     const long long l0 = (long long)blockIdx.x * G;
     const int g_here = (int)((a.nlines - l0) < G ? (a.nlines - l0) : G);
     const C *src = (const C *)a.in;
+    const C *twp = (const C *)a.tw;
+    if constexpr (TWL) {
+        C *tl = B + G * L;
+        for (int i = threadIdx.x; i < L; i += blockDim.x) tl[i] = twp[i];
+        twp = tl;
+    }
     const float iL = rcp_up(L);
     for (int idx = threadIdx.x; idx < G * L; idx += blockDim.x) {
         const int g = fdiv(idx, iL), p = idx - __mul24(g, L);
         A[idx] = g < g_here ? src[(l0 + g) * a.in_dist + p] : cmk<C>((T)0, (T)0);
     }
     __syncthreads();
-    const C *res = smooth_lines<T, DIR>(A, B, G, a.sp, (const C *)a.tw);
+    const C *res = smooth_lines<T, DIR>(A, B, G, a.sp, twp);
     C *dst = (C *)a.out;
     const T sc = (T)a.scale;
     for (int idx = threadIdx.x; idx < G * L; idx += blockDim.x) {
