@@ -140,6 +140,13 @@ const char *nfft_last_error(void);
 int nfft_fill_uniform(void *out, long long count, unsigned long long seed,
                       unsigned long long start, int prec, void *stream);
 
+/* Same stream, Gaussian N(0, 1) by Box-Muller: value i (j = start + i) is
+ * sqrt(-2 ln u1) cos(2 pi u2) for even j, sqrt(-2 ln u1) sin(2 pi u2) for odd
+ * j, with u1 = V(2 (j/2)), u2 = V(2 (j/2) + 1) and V(k) = ((z_k >> 11) + 1) 2^-53
+ * in (0, 1] the SplitMix64 draw k of `seed`, computed in FP64. */
+int nfft_fill_gaussian(void *out, long long count, unsigned long long seed,
+                       unsigned long long start, int prec, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

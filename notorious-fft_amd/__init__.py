@@ -91,6 +91,8 @@ def core():
         lib.nfft_last_error.argtypes = []
         lib.nfft_fill_uniform.restype = _i
         lib.nfft_fill_uniform.argtypes = [_vp, _ll, ctypes.c_ulonglong, ctypes.c_ulonglong, _i, _vp]
+        lib.nfft_fill_gaussian.restype = _i
+        lib.nfft_fill_gaussian.argtypes = [_vp, _ll, ctypes.c_ulonglong, ctypes.c_ulonglong, _i, _vp]
         lib.nfft_prof_enable.restype = _i
         lib.nfft_prof_enable.argtypes = [_i, _i]
         lib.nfft_prof_count.restype = _i
@@ -198,6 +200,11 @@ class Plan:
 def fill_uniform(ptr, count, seed, start=0, prec="f32", stream=0):
     check(core().nfft_fill_uniform(ptr, count, seed, start, F32 if prec == "f32" else F64, stream),
           "fill_uniform")
+
+
+def fill_gaussian(ptr, count, seed, start=0, prec="f32", stream=0):
+    check(core().nfft_fill_gaussian(ptr, count, seed, start, F32 if prec == "f32" else F64, stream),
+          "fill_gaussian")
 
 
 def prof_enable(on=True, capacity=8192):

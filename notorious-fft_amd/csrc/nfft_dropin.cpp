@@ -11,7 +11,8 @@
 //   - invdft is unnormalised for power-of-two lengths and scaled by 1/N along
 //     non-power-of-two axes (src/05_algorithms.h:315-333);
 //   - transforms are synchronous, never modify their input, allow x == y,
-//     and do nothing when handed a NULL aux;
+//     and do nothing when handed a NULL aux; work the caller queued on the
+//     device before the call is complete before the transform reads x;
 //   - free_aux(NULL) is a no-op (src/07_legacy.h:1690-1691).
 // Multi-dimensional realdft, declared but not implemented by the reference
 // (src/07_legacy.h:1492-1514), follows FFTW's layout: the last axis is halved
@@ -110,6 +111,12 @@ void run(const notorious_fft_aux *a, int op, const void *x, void *y)
     Impl *im = (Impl *)a->impl;
     size_t ib, ob;
     op_bytes(im, op, &ib, &ob);
+    // The reference's calls are synchronous on the caller's thread, so any
+    // write of x (or read of y) the caller has already queued on a device
+    // stream -- a kernel, an async copy on the null stream or on a stream of
+    // its own -- happens before this transform.  The aux's stream is
+    // non-blocking, so that order is established here for every pointer kind.
+    if (hipDeviceSynchronize() != hipSuccess) return;
     const bool xd = on_device(x), yd = on_device(y);
     if (xd && yd) {
         if (nfft_execute(im->plan, op, x, y, im->st) == NFFT_OK) (void)hipStreamSynchronize(im->st);

@@ -134,6 +134,8 @@ hipError_t launch_blue_hprep(const double2 *H, void *out, int prec, int m, int m
                              hipStream_t s);
 hipError_t launch_fill_uniform(void *out, long long count, uint64_t seed, uint64_t start, int prec,
                                hipStream_t s);
+hipError_t launch_fill_gaussian(void *out, long long count, uint64_t seed, uint64_t start, int prec,
+                               hipStream_t s);
 // strided transpose of a batch of [rows x cols] matrices of elements of
 // `esz` bytes (4, 8 or 16): out[b][c][r] = in[b][r][c]
 hipError_t launch_transpose(const void *in, void *out, long long batch, long long rows, long long cols, int esz,

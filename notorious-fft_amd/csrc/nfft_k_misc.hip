@@ -240,9 +240,47 @@ __global__ void k_fill_uniform(void *out, long long count, unsigned long long se
 hipError_t launch_fill_uniform(void *out, long long count, uint64_t seed, uint64_t start, int prec, hipStream_t s)
 {
     if (count <= 0) return hipSuccess;
-    long long blocks = (count + 255) / 256;
-    // one launch per 2^31 blocks is far beyond any buffer here; keep it simple
+    const long long blocks = (count + 255) / 256;
+    if (blocks > 0xffffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_fill_uniform, dim3((unsigned)blocks), dim3(256), 0, s, out, count, seed, start, prec);
+    return hipGetLastError();
+}
+
+// Gaussian N(0, 1) by Box-Muller on consecutive uniforms of the same stream:
+// value j (j = start + i) belongs to pair p = j / 2 with u1 = U(2p), u2 =
+// U(2p + 1), U(k) = ((z_k >> 11) + 1) 2^-53 in (0, 1]; it is
+// sqrt(-2 ln u1) cos(2 pi u2) for even j and sqrt(-2 ln u1) sin(2 pi u2) for
+// odd j (FP64, rounded once to the output type).
+__device__ __forceinline__ double sm64_unit_open0(unsigned long long seed, unsigned long long idx)
+{
+    unsigned long long z = ((seed << 40) | idx) + 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    z = z ^ (z >> 31);
+    return (double)((z >> 11) + 1) * (1.0 / 9007199254740992.0);
+}
+
+__global__ void k_fill_gaussian(void *out, long long count, unsigned long long seed, unsigned long long start,
+                                int prec)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const unsigned long long j = start + (unsigned long long)i, p = j >> 1;
+    const double u1 = sm64_unit_open0(seed, 2 * p), u2 = sm64_unit_open0(seed, 2 * p + 1);
+    const double r = sqrt(-2.0 * log(u1));
+    double sn, cs;
+    sincospi(2.0 * u2, &sn, &cs);
+    const double v = r * ((j & 1) ? sn : cs);
+    if (prec == F32) ((float *)out)[i] = (float)v;
+    else ((double *)out)[i] = v;
+}
+
+hipError_t launch_fill_gaussian(void *out, long long count, uint64_t seed, uint64_t start, int prec, hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    const long long blocks = (count + 255) / 256;
+    if (blocks > 0xffffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_fill_gaussian, dim3((unsigned)blocks), dim3(256), 0, s, out, count, seed, start, prec);
     return hipGetLastError();
 }
 

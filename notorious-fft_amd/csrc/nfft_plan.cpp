@@ -1475,4 +1475,20 @@ int nfft_fill_uniform(void *out, long long count, unsigned long long seed, unsig
     return NFFT_OK;
 }
 
+int nfft_fill_gaussian(void *out, long long count, unsigned long long seed, unsigned long long start, int prec,
+                       void *stream)
+{
+    clear_error();
+    if (!out || count < 0 || (prec != NFFT_F32 && prec != NFFT_F64)) {
+        set_error("invalid fill arguments");
+        return NFFT_E_INVALID;
+    }
+    hipError_t e = launch_fill_gaussian(out, count, seed, start, prec, (hipStream_t)stream);
+    if (e != hipSuccess) {
+        set_error(hipGetErrorString(e));
+        return NFFT_E_HIP;
+    }
+    return NFFT_OK;
+}
+
 }  // extern "C"

@@ -43,3 +43,32 @@ def run_plan(pkg, dev, op, dims, x, batch=1, prec="f32", norm="reference", vjp=F
 def rounded(x, prec):
     """The FP64 view of the values the device actually received."""
     return np.asarray(x).astype(np_dtype(prec)).astype(np.float64)
+
+
+def pocket(op, dims, x, batch=1):
+    """numpy.fft (pocketfft, FP64) reference for dft / invdft / realdft /
+    invrealdft over dims with the reference's normalisation: invdft is
+    unnormalised along power-of-two axes and scaled by 1/N along the others
+    (src/05_algorithms.h:315-333, also for the leading axes of invrealdft);
+    realdft / invrealdft are unnormalised (src/07_legacy.h:162-293).
+    Returns interleaved reals like the device output."""
+    dims = list(dims)
+    x = np.asarray(x, dtype=np.float64)
+    axes = tuple(range(1, len(dims) + 1))
+    pow2 = float(np.prod([n for n in dims if n & (n - 1) == 0]))
+    if op in ("dft", "invdft", "invrealdft"):
+        if op == "invrealdft":
+            h = dims[:-1] + [dims[-1] // 2 + 1]
+            z = (x[0::2] + 1j * x[1::2]).reshape([batch] + h)
+            return (np.fft.irfftn(z, s=dims, axes=axes) * pow2).reshape(-1)
+        z = (x[0::2] + 1j * x[1::2]).reshape([batch] + dims)
+        out = np.fft.fftn(z, axes=axes) if op == "dft" else np.fft.ifftn(z, axes=axes) * pow2
+    elif op == "realdft":
+        out = np.fft.rfftn(x.reshape([batch] + dims), axes=axes)
+    else:
+        raise ValueError(op)
+    out = out.reshape(-1)
+    r = np.empty(2 * out.size)
+    r[0::2] = out.real
+    r[1::2] = out.imag
+    return r

@@ -117,6 +117,28 @@ def uniform_np(seed, count, start=0):
     return 2.0 * ((z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)) - 1.0
 
 
+def gaussian_np(seed, count, start=0):
+    """Box-Muller N(0, 1) over the same stream (the device generator's
+    definition): value j = start + i uses the pair p = j // 2 of uniforms
+    u1 = V(2p), u2 = V(2p + 1) in (0, 1], V(k) = ((z_k >> 11) + 1) 2^-53;
+    sqrt(-2 ln u1) cos(2 pi u2) for even j, sin for odd j."""
+    j = np.arange(start, start + count, dtype=np.uint64)
+    p = j >> np.uint64(1)
+
+    def v(k):
+        with np.errstate(over="ignore"):
+            z = ((np.uint64(seed) << np.uint64(40)) | k) + np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        return ((z >> np.uint64(11)) + np.uint64(1)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+    u1 = v(np.uint64(2) * p)
+    u2 = v(np.uint64(2) * p + np.uint64(1))
+    r = np.sqrt(-2.0 * np.log(u1))
+    return np.where((j & np.uint64(1)) == 0, r * np.cos(2 * np.pi * u2), r * np.sin(2 * np.pi * u2))
+
+
 class Ref:
     """The reference library compiled from its own header (oracle/_ref)."""
 
@@ -186,8 +208,19 @@ class Ref:
         return y, t.value
 
 
+def as_reals(x):
+    """Flatten to float64 reals; complex input becomes interleaved (re, im)
+    pairs, so that both parts take part in a comparison."""
+    x = np.asarray(x)
+    if np.iscomplexobj(x):
+        x = np.stack([x.real, x.imag], axis=-1)
+    return np.asarray(x, dtype=np.float64).reshape(-1)
+
+
 def rel_l2(got, ref):
-    got = np.asarray(got, dtype=np.float64).reshape(-1)
-    ref = np.asarray(ref, dtype=np.float64).reshape(-1)
+    got = as_reals(got)
+    ref = as_reals(ref)
+    if got.size != ref.size:
+        raise ValueError(f"rel_l2: sizes differ ({got.size} vs {ref.size})")
     den = np.linalg.norm(ref)
     return float(np.linalg.norm(got - ref) / (den if den > 0 else 1.0))

@@ -169,6 +169,34 @@ def test_dropin_f64_device_pointers(pkg, dev):
     assert ol.rel_l2(yd.cpu().numpy(), ol.run_oracle("dft", [n], x)) < tol_for("f64", n)
 
 
+@pytest.mark.parametrize("side_stream", [False, True])
+def test_dropin_orders_after_queued_writes(pkg, dev, side_stream):
+    """The caller queues the write of x (a copy behind a long chain of its own
+    kernels, on torch's current stream or on a stream of its own) and calls
+    the drop-in with no synchronisation: the synchronous reference contract
+    says the transform sees the written x."""
+    import torch
+    lib = pkg.dropin("f64")
+    n = 4096
+    x = ol.uniform(5, 2 * n)
+    src = torch.from_numpy(x).to(dev)
+    xd = torch.zeros_like(src)
+    yd = torch.zeros_like(src)
+    a = lib.notorious_fft_mkaux_dft_1d(n)
+    busy = torch.ones(1 << 24, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    st = torch.cuda.Stream() if side_stream else torch.cuda.current_stream()
+    with torch.cuda.stream(st):
+        for _ in range(40):
+            busy.mul_(1.0000001)
+        xd.copy_(src)
+    lib.notorious_fft_dft(xd.data_ptr(), yd.data_ptr(), a)
+    got = yd.cpu().numpy()
+    torch.cuda.synchronize()
+    lib.notorious_fft_free_aux(a)
+    assert ol.rel_l2(got, ol.run_oracle("dft", [n], x)) < tol_for("f64", n)
+
+
 def test_dropin_f32(pkg, dev):
     lib = pkg.dropin("f32")
     for n in (16, 1024, 100):

@@ -1,11 +1,17 @@
 """GPU parity of the N-D transforms (row-major, last index fastest) against
 the oracle's tree recursion (src/07_legacy.h:109-140 and the DCT/DST N-D
-loops), including non-power-of-two axes, unit axes and batches."""
+loops), including non-power-of-two axes, unit axes and batches.
+
+FP64 cases with a non-power-of-two axis are judged against pocketfft
+(numpy.fft) at the FP64 tolerance: the oracle restates the reference's
+Bluestein, whose chirp phase pi k^2 / N is not reduced mod 2N
+(src/06_plan.h:91-112) and carries an error growing with N; the oracle
+comparison stays as a parity cross-check at the width of that error."""
 import numpy as np
 import pytest
 
 import oracle_lib as ol
-from gpu_util import rounded, run_plan, tol_for
+from gpu_util import pocket, rounded, run_plan, tol_for
 
 pytestmark = pytest.mark.gpu
 
@@ -24,9 +30,14 @@ def test_dft_nd(pkg, dev, prec, dims):
         ref = ol.run_oracle(op, dims, x, batch)
         err = ol.rel_l2(got, ref)
         blue = any(n & (n - 1) for n in dims)
-        tol = tol_for(prec, p) * (1 if prec == "f32" or not blue else 1e4)
         print(f"{prec} {op} {dims}: rel-L2 {err:.3e}")
-        assert err < tol, (op, err)
+        if prec == "f64" and blue:
+            errp = ol.rel_l2(got, pocket(op, dims, x, batch))
+            print(f"    vs pocketfft {errp:.3e}")
+            assert errp < tol_for(prec, p), (op, errp)
+            assert err < tol_for(prec, p) * 1e4, (op, err)
+        else:
+            assert err < tol_for(prec, p), (op, err)
 
 
 REAL_DIMS = [[4, 8], [8, 16], [3, 8], [4, 2, 8], [64, 128], [16, 16, 16], [1, 8], [8, 1], [4096, 64]]
@@ -48,9 +59,15 @@ def test_realdft_nd(pkg, dev, prec, dims):
     erri = ol.rel_l2(goti, refi)
     print(f"{prec} realdft {dims}: fwd {err:.3e} inv {erri:.3e}")
     blue = any(n & (n - 1) for n in dims)
-    k = 1e4 if (prec == "f64" and blue) else 1
-    assert err < tol_for(prec, p) * k
-    assert erri < tol_for(prec, p) * k
+    if prec == "f64" and blue:
+        errp = ol.rel_l2(got, pocket("realdft", dims, x, batch))
+        errpi = ol.rel_l2(goti, pocket("invrealdft", dims, z, batch))
+        print(f"    vs pocketfft: fwd {errp:.3e} inv {errpi:.3e}")
+        assert errp < tol_for(prec, p) and errpi < tol_for(prec, p)
+        assert err < tol_for(prec, p) * 1e4 and erri < tol_for(prec, p) * 1e4
+    else:
+        assert err < tol_for(prec, p)
+        assert erri < tol_for(prec, p)
 
 
 TRIG_DIMS = [[4, 4], [8, 16], [4, 8, 8], [16, 1], [1, 16], [1, 1], [64, 256], [2, 1, 4]]

@@ -46,6 +46,20 @@ def test_rng_matches_numpy_restatement():
     assert abs(a.mean()) < 0.1
 
 
+def test_rel_l2_sees_both_parts():
+    """A complex comparison must see the imaginary part: a conjugation, a
+    sign flip of the imaginary half or a swap of the parts is a large
+    error, as it is for the interleaved float view of the same data."""
+    rng = np.random.default_rng(0)
+    z = rng.standard_normal(500) + 1j * rng.standard_normal(500)
+    assert ol.rel_l2(np.conj(z), z) > 0.5
+    assert ol.rel_l2(1j * np.conj(z), z) > 0.5
+    assert ol.rel_l2(z.astype(np.complex64), z) < 1e-6
+    assert abs(ol.rel_l2(np.conj(z), z) - ol.rel_l2(il(np.conj(z)), il(z))) < 1e-15
+    with pytest.raises(ValueError):
+        ol.rel_l2(z, il(z)[:10])
+
+
 def test_golden_vectors_bitwise():
     man = json.load(open(MANIFEST))
     data = np.load(GOLDEN, allow_pickle=False)
