@@ -391,6 +391,66 @@ __device__ __forceinline__ void line_fft(Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds, 
     line_fft_tw<T, L, DIR>(reg, lds, lay, j, StageTwGlobal{tw});
 }
 
+// NL independent lines per thread (line l: registers reg[l], LDS layout
+// lay[l], thread index j[l] within its line), transformed together so that
+// they share the block barriers of each exchange: a workgroup of NT * LPB / NL
+// threads covers a tile of LPB lines, each thread holding NL x 16 points.
+// Same arithmetic, in the same order per line, as line_fft_tw.
+template <typename T, int L, int DIR, int NL, class Lay, class Tw>
+__device__ __forceinline__ void line_fft_tw_n(Cx<T> (&reg)[NL][Sched<L>::R], Cx<T> *lds, const Lay (&lay)[NL],
+                                              const int (&j)[NL], const Tw &twsrc)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    typename Tw::B bw[NL][4];
+    sfor<0, S::S>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr int Rs = S::radix(s), Ns = S::ns(s), m = S::R / Rs;
+#pragma unroll
+        for (int l = 0; l < NL; l++) {
+            if constexpr (s == 0 && S::S > 1 && Tw::kAhead)
+                twsrc.load(S::twoff(1) + (j[l] & (S::ns(1) - 1)) * 4, bw[l]);
+            if constexpr (s > 0 && !Tw::kAhead) twsrc.load(S::twoff(s) + (j[l] & (S::ns(s) - 1)) * 4, bw[l]);
+#pragma unroll
+            for (int i = 0; i < m; i++) {
+                C a[Rs];
+#pragma unroll
+                for (int r = 0; r < Rs; r++) a[r] = reg[l][i + r * m];
+                if constexpr (s > 0) {
+                    static_assert(Rs == 16 && m == 1, "twiddled stages are radix 16");
+                    apply_stage_tw<DIR>(a, bw[l]);
+                }
+                dftR<Rs, DIR>(a);
+#pragma unroll
+                for (int r = 0; r < Rs; r++) reg[l][i + r * m] = a[r];
+            }
+            // one set's butterflies at a time: interleaved, the sets' temporaries
+            // would be live together
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if constexpr (s + 1 < S::S) {
+#pragma unroll
+            for (int l = 0; l < NL; l++) {
+                if constexpr (s > 0 && Tw::kAhead)
+                    twsrc.load(S::twoff(s + 1) + (j[l] & (S::ns(s + 1) - 1)) * 4, bw[l]);
+#pragma unroll
+                for (int i = 0; i < m; i++) {
+                    const int v = j[l] + S::NT * i;
+                    const int base = (v / Ns) * Ns * Rs + (v & (Ns - 1));
+#pragma unroll
+                    for (int r = 0; r < Rs; r++) lds[lay[l](base + r * Ns)] = reg[l][i + r * m];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int l = 0; l < NL; l++)
+#pragma unroll
+                for (int q = 0; q < S::R; q++) reg[l][q] = lds[lay[l](j[l] + S::NT * q)];
+            __syncthreads();
+        }
+    });
+}
+
 // Twiddle for the four-step / Bluestein inter-pass products: w_N^{e},
 // e < N, from two FP64 tables (hi: e >> sh, lo: e & mask), product in FP64.
 struct Tw2 {
@@ -487,27 +547,29 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc_if(const void *base, 
     const int n = __builtin_amdgcn_readfirstlane(live ? -1 : 0);
     return __builtin_amdgcn_make_buffer_rsrc((void *)(((uint64_t)hi << 32) | lo), (short)0, n, 0x00020000);
 }
-template <typename C> __device__ __forceinline__ C buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
+// CP: cache-policy bits of the access (0: default; 2: non-temporal).
+template <typename C, int CP = 0>
+__device__ __forceinline__ C buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff)
 {
     if constexpr (sizeof(C) == 8)
-        return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+        return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, CP));
     else
-        return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+        return __builtin_bit_cast(C, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, CP));
 }
 // 16-byte stores take the whole offset in the lane register (scalar offset
 // 0): the compiler's wait state between a store of more than 8 bytes and a
 // following write of its data registers is only inserted for that form, and
 // FP64 results written through the scalar-offset form were intermittently
 // corrupted when other kernels ran at the same time.
-template <typename C>
+template <typename C, int CP = 0>
 __device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff, C v)
 {
     if constexpr (sizeof(C) == 8)
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v), r,
-                                              (int)voff, (int)soff, 0);
+                                              (int)voff, (int)soff, CP);
     else
         __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, v),
-                                               r, (int)(voff + soff), 0, 0);
+                                               r, (int)(voff + soff), 0, CP);
 }
 
 // Logical workgroup index for XCD-aware tiling: workgroups are dispatched

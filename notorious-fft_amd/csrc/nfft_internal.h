@@ -9,6 +9,7 @@
 
 #include "nfft_kernels.h"
 #include "nfft_kernels_persist.h"
+#include "nfft_kernels_xcd.h"
 #include "nfft_kernels_smooth.h"
 #include "nfft_kernels_rt.h"
 
@@ -105,6 +106,16 @@ bool persist_find(int prec, long long n, int variant, PersistCfg *cfg);
 hipError_t launch_four_persist(const PersistCfg &c, int dir, const PersistArgs &a, int grid, hipStream_t s);
 hipError_t launch_persist_reset(unsigned *ctr, long long words, hipStream_t s);
 
+// XCD-affine FP32 four-step (nfft_k_xcd.hip): true if length n has an
+// instance (N1 x 1024); n1 and the resident workgroups per CU are returned.
+bool xcd_supported(long long n, int *n1, int *per_cu);
+// The XCC ids on which the current device runs workgroups, as queue numbers
+// 0..nq-1 (a one-time probe launch per device); false if it has more XCDs
+// than the kernel has queues.
+bool xcd_topology(int *nq, signed char *qmap);
+// twrow: inter-pass twiddle in the row tiles (FP32 table) or the column tiles
+hipError_t launch_four_xcd(int n1, int dir, bool twrow, const XcdArgs &a, int grid, hipStream_t s);
+
 // mixed-radix lines (nfft_k_smooth.hip); smooth_factor() builds the stage
 // schedule of a 7-smooth length (false otherwise)
 bool smooth_factor(long long n, SmoothPlan *sp);
@@ -188,6 +199,13 @@ struct C2C {
     int pgrab = 1;                    // queue items taken per access
     int plag = 0;                     // pass-2 lag in transforms (0: half the slots)
     long long pslots = 1;             // rotating slots at most (Infinity Cache budget)
+    // FP32 ALG_FOUR in one XCD-affine launch (k_four_xcd) when xctr is set
+    // and a batch has at least kXcdQueues * kXcdSlots transforms
+    unsigned *xctr = nullptr;         // owned: counter block for up to xcap transforms
+    long long xcap = 0;
+    int xgrid = 0;                    // workgroups of the launch
+    int xnq = 0;                      // XCD queues and the XCC_ID -> queue map
+    signed char xqmap[kXcdIds] = {};
 
     // Set up tables for length n; returns false with the error set.
     // smooth_mode for 7-smooth non-power-of-two n: 2 = choose (tuned /

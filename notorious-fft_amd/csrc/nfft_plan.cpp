@@ -377,6 +377,51 @@ static bool persist_setup(C2C &c)
     return true;
 }
 
+// Transforms per XCD-affine launch (bounds its counter block: 24 words per
+// transform).
+static const long long kXcdCap = 1LL << 12;
+
+// The XCD-affine four-step for FP32 lengths that have an instance, where it
+// is the default: N = 2^20 (2^28 points: 1.62-1.63 ms vs 1.75 ms for the two
+// launches; 2^19 1.61 vs 1.61 ms and 2^21 1.80-1.82 vs 1.75 ms keep the two
+// launches).  NFFT_XCD=0 turns it off, NFFT_XCD=2 uses it for every length
+// that has an instance.
+static bool xcd_setup(C2C &c)
+{
+    int n1 = 0, per_cu = 0;
+    const int mode = env_int("NFFT_XCD", 1);
+    if (c.prec != F32 || mode == 0 || (mode == 1 && c.n != (1 << 20)) || !xcd_supported(c.n, &n1, &per_cu))
+        return false;
+    int dev = 0, ncu = 0, khz = 0;
+    if (!hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
+        !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "device attribute") ||
+        !hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "device attribute"))
+        return false;
+    const size_t bytes = (size_t)xcd_ctr_words(kXcdCap) * sizeof(unsigned);
+    if (!hip_ok(hipMalloc(&c.xctr, bytes), "hipMalloc")) return false;
+    if (!hip_ok(hipMemset(c.xctr, 0, bytes), "hipMemset")) {
+        (void)hipFree(c.xctr);
+        c.xctr = nullptr;
+        return false;
+    }
+    c.xcap = kXcdCap;
+    c.xgrid = std::max(1, ncu) * per_cu;
+    if (c.xgrid > kXcdMaxGrid || kXcdCap * std::max(XShape<512>::B, XShape<2048>::A) > kXcdMaxTiles) {
+        (void)hipFree(c.xctr);
+        c.xctr = nullptr;
+        return false;
+    }
+    if (!xcd_topology(&c.xnq, c.xqmap)) {
+        (void)hipFree(c.xctr);
+        c.xctr = nullptr;
+        return false;
+    }
+    c.ptimeout = (long long)(kPersistTimeoutSec * 1000.0 * std::max(khz, 1000));
+    c.n1 = n1;
+    c.n2 = (int)(c.n / n1);
+    return true;
+}
+
 bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
 {
     n = n_;
@@ -425,6 +470,12 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
             // 992 -> 910 us per 2^28 points; the prefetching row grid absorbs it)
             row_twin = prec == F32 && n2 == 1024 && env_int("NFFT_ROW_TWIDDLE", 1) != 0 ? 1 : 0;
             if (row_twin && !tw2f_table(n, &tw2f)) row_twin = 0;
+            // batches of at least 16 transforms: the XCD-affine single launch
+            // (same N1 x 1024 split, inter-pass twiddle where row_twin puts it)
+            if (persist_mode != 0 && n2 == 1024 && xcd_setup(*this) && (n1 != n / 1024 || n2 != 1024)) {
+                set_error("internal: XCD four-step split");
+                return false;
+            }
         }
         tw_a = stage_table(prec, n1);
         tw_b = stage_table(prec, n2);
@@ -526,14 +577,19 @@ void C2C::release()
     H = nullptr;
     if (pctr) (void)hipFree(pctr);
     pctr = nullptr;
+    if (xctr) (void)hipFree(xctr);
+    xctr = nullptr;
 }
 
 bool C2C::device_fault() const
 {
-    if (!pctr) return false;
-    unsigned w[2] = {0, 0};
-    if (hipMemcpy(w, pctr + 1, sizeof w, hipMemcpyDeviceToHost) != hipSuccess) return true;
-    return (w[0] | w[1]) != 0;
+    for (unsigned *c : {pctr, xctr}) {
+        if (!c) continue;
+        unsigned w[2] = {0, 0};
+        if (hipMemcpy(w, c + 1, sizeof w, hipMemcpyDeviceToHost) != hipSuccess) return true;
+        if ((w[0] | w[1]) != 0) return true;
+    }
+    return false;
 }
 
 std::string C2C::describe() const
@@ -545,6 +601,9 @@ std::string C2C::describe() const
         if (pctr)
             snprintf(b, sizeof b, "four-step %dx%d persistent (tiles %d/%d, %d workgroups)", n1, n2, pcfg.lpb1,
                      pcfg.lpb2, pgrid);
+        else if (xctr)
+            snprintf(b, sizeof b, "four-step %dx%d XCD-affine (%d workgroups; two launches below 16 transforms)",
+                     n1, n2, xgrid);
         else
             snprintf(b, sizeof b, "four-step %dx%d", n1, n2);
         break;
@@ -631,6 +690,34 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
                 const int grid = (int)std::min<long long>(pgrid, (items + pgrab - 1) / pgrab);
                 hipError_t e = launch_persist_reset(pctr, persist_ctr_words(cnt), s);
                 if (e == hipSuccess) e = launch_four_persist(pcfg, dir, a, grid, s);
+                if (e != hipSuccess) return e;
+            }
+            return hipSuccess;
+        }
+        constexpr long long kXcdLines = (long long)kXcdQueues * kXcdSlots;
+        if (xctr && nlines >= kXcdLines && scratch_lines >= (long long)xnq * kXcdSlots) {
+            // one XCD-affine launch per xcap transforms; the scratch holds the
+            // two intermediate slots of each XCD
+            for (long long l0 = 0; l0 < nlines; l0 += xcap) {
+                const long long cnt = std::min(xcap, nlines - l0);
+                XcdArgs a{};
+                a.in = (const char *)in + (size_t)l0 * in_dist * cb;
+                a.out = (char *)out + (size_t)l0 * out_dist * cb;
+                a.ntrans = cnt;
+                a.in_dist = in_dist;
+                a.out_dist = out_dist;
+                a.scratch = scratch;
+                a.ctr = xctr;
+                a.tw_a = tw_a;
+                a.tw_b = tw_b;
+                a.twf = tw2f;
+                a.tw2 = tw2;
+                a.scale = scale;
+                a.timeout = ptimeout;
+                a.nq = xnq;
+                for (int i = 0; i < kXcdIds; i++) a.qmap[i] = xqmap[i];
+                hipError_t e = launch_persist_reset(xctr, xcd_ctr_words(cnt), s);
+                if (e == hipSuccess) e = launch_four_xcd(n1, dir, row_twin != 0, a, xgrid, s);
                 if (e != hipSuccess) return e;
             }
             return hipSuccess;
