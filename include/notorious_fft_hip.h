@@ -72,6 +72,19 @@ int nfft_plan_destroy(nfft_plan *plan);
  * INVREALDFT with batch > 1.  The input is never modified. */
 int nfft_execute(nfft_plan *plan, int op, const void *in, void *out, void *stream);
 
+/* nfft_execute for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops): dft / invdft of a DFT plan with
+ * every kernel's scratch (four-step intermediates, Bluestein passes, the
+ * full-array copy of a strided axis) taken from one static device array of
+ * 64 MiB per GPU, held from the library's first use on that GPU until the
+ * process ends.  No counterpart in the reference.  The call allocates
+ * nothing; results are those of nfft_execute (the batch is worked through in
+ * chunks that fit).  Plans needing more than 64 MiB for one chunk (e.g. a
+ * strided axis whose full array exceeds it) are rejected with NFFT_E_SIZE.
+ * All calls on one GPU share the array: calls that overlap in time on one
+ * GPU (other threads, other streams without ordering) race. */
+int nfft_execute_noalloc(nfft_plan *plan, int op, const void *in, void *out, void *stream);
+
 /* Vector-Jacobian product of `op`: grad_in = J^T grad_out, where complex
  * gradients use the convention g = dL/dRe + i dL/dIm.  Layouts are those of
  * the op's output (grad_out) and input (grad_in).

@@ -81,6 +81,8 @@ def core():
         lib.nfft_execute.argtypes = [_vp, _i, _vp, _vp, _vp]
         lib.nfft_vjp.restype = _i
         lib.nfft_vjp.argtypes = [_vp, _i, _vp, _vp, _vp]
+        lib.nfft_execute_noalloc.restype = _i
+        lib.nfft_execute_noalloc.argtypes = [_vp, _i, _vp, _vp, _vp]
         lib.nfft_plan_workspace_bytes.restype = _ll
         lib.nfft_plan_workspace_bytes.argtypes = [_vp]
         lib.nfft_plan_describe.restype = ctypes.c_char_p
@@ -170,6 +172,10 @@ class Plan:
 
     def execute(self, op, src_ptr, dst_ptr, stream=0):
         check(core().nfft_execute(self.handle, OP[op], src_ptr, dst_ptr, stream), f"execute {op}")
+
+    def execute_noalloc(self, op, src_ptr, dst_ptr, stream=0):
+        """nfft_execute_noalloc: the library's static scratch instead of the plan's."""
+        check(core().nfft_execute_noalloc(self.handle, OP[op], src_ptr, dst_ptr, stream), f"execute_noalloc {op}")
 
     def vjp(self, op, g_ptr, out_ptr, stream=0):
         check(core().nfft_vjp(self.handle, OP[op], g_ptr, out_ptr, stream), f"vjp {op}")

@@ -29,6 +29,9 @@ namespace nfft {
 // errors
 // ---------------------------------------------------------------------------
 static thread_local std::string t_err;
+// Set for the duration of an allocation-free execute: engines then use only
+// the scratch they are handed (no plan-owned counter blocks or slots).
+static thread_local int t_noalloc = 0;
 void set_error(const std::string &msg) { t_err = msg; }
 void clear_error() { t_err.clear(); }
 const char *last_error() { return t_err.c_str(); }
@@ -665,7 +668,7 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
     }
     case ALG_FOUR: {
         if (scratch_lines < 1 || !scratch) return hipErrorInvalidValue;
-        if (pctr) {
+        if (pctr && !t_noalloc) {
             // one persistent launch per kPersistCap transforms; the scratch
             // holds the rotating intermediate slots
             for (long long l0 = 0; l0 < nlines; l0 += pcap) {
@@ -695,7 +698,7 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
             return hipSuccess;
         }
         constexpr long long kXcdLines = (long long)kXcdQueues * kXcdSlots;
-        if (xctr && nlines >= kXcdLines && scratch_lines >= (long long)xnq * kXcdSlots) {
+        if (xctr && !t_noalloc && nlines >= kXcdLines && scratch_lines >= (long long)xnq * kXcdSlots) {
             // one XCD-affine launch per xcap transforms; the scratch holds the
             // two intermediate slots of each XCD
             for (long long l0 = 0; l0 < nlines; l0 += xcap) {
@@ -1368,6 +1371,59 @@ int nfft_plan::execute(int op, const void *in, void *out, hipStream_t s, int vjp
     return NFFT_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (nfft_execute_noalloc): one
+// array per GPU the library runs on, held until the process ends.  Host code
+// uses only the device address obtained for the current device, never the
+// array's name.
+// ---------------------------------------------------------------------------
+static constexpr size_t kStaticScratchBytes = 64u << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::once_flag g_scratch_once[kMaxDevices];
+void *g_scratch_ptr[kMaxDevices];
+}  // namespace
+
+// Device address of `scratch` on the current device (resolved once per
+// device); nullptr with the error set on failure.
+static void *static_scratch()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error("no usable current device for the static scratch");
+        return nullptr;
+    }
+    std::call_once(g_scratch_once[dev], [dev] {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[dev] = p;
+    });
+    if (!g_scratch_ptr[dev]) set_error("hipGetSymbolAddress of the static scratch failed");
+    return g_scratch_ptr[dev];
+}
+
+// Scratch elements (complex) an execute of DFT plan p needs when it may chunk
+// freely: one line of each engine that needs scratch, plus the full-array copy
+// of a strided axis that is not transformed in place.
+static long long dft_min_scratch(const nfft_plan &p)
+{
+    long long need = 0;
+    const C2C &el = p.ax[p.d - 1];
+    if (p.dims[p.d - 1] > 1) need = el.scratch_per_line;
+    for (int i = 0; i + 1 < p.d; i++) {
+        const C2C &e = p.ax[i];
+        if (p.dims[i] == 1) continue;
+        const long long B = prod(p.dims, i + 1, p.d);
+        if (B == 1) {
+            need = std::max(need, e.scratch_per_line);
+        } else if (!(e.algo == ALG_LINE && p.dims[i] <= col_max(p.prec))) {
+            need = std::max(need, p.batch * p.P + e.scratch_per_line);
+        }
+    }
+    return need;
+}
+
 // ===========================================================================
 // C ABI
 // ===========================================================================
@@ -1412,6 +1468,9 @@ int nfft_plan_create(nfft_plan **plan, int kind, int prec, int d, const int *dim
                 return NFFT_E_SIZE;
             }
     }
+    // the allocation-free entry point's scratch address, resolved at set-up
+    // on this device (an execute under graph capture then only reads it)
+    if (kind == NFFT_KIND_DFT && !static_scratch()) return NFFT_E_HIP;
     nfft_plan *pl = new (std::nothrow) nfft_plan();
     if (!pl) {
         set_error("out of host memory");
@@ -1446,6 +1505,40 @@ int nfft_execute(nfft_plan *plan, int op, const void *in, void *out, void *strea
         return NFFT_E_INVALID;
     }
     return plan->execute(op, in, out, (hipStream_t)stream, 0);
+}
+
+int nfft_execute_noalloc(nfft_plan *plan, int op, const void *in, void *out, void *stream)
+{
+    clear_error();
+    if (!plan || !in || !out) {
+        set_error("NULL plan or buffer");
+        return NFFT_E_INVALID;
+    }
+    if (plan->kind != NFFT_KIND_DFT || (op != NFFT_OP_DFT && op != NFFT_OP_INVDFT)) {
+        set_error("nfft_execute_noalloc runs dft / invdft of DFT plans only");
+        return NFFT_E_INVALID;
+    }
+    const size_t cb = cx_bytes(plan->prec);
+    const long long need = dft_min_scratch(*plan);
+    if ((size_t)need * cb > kStaticScratchBytes) {
+        set_error("this plan needs " + std::to_string((size_t)need * cb) +
+                  " bytes of scratch; the allocation-free entry point has " + std::to_string(kStaticScratchBytes));
+        return NFFT_E_SIZE;
+    }
+    void *scr = static_scratch();
+    if (!scr) return NFFT_E_HIP;
+    struct Guard {
+        Guard() { t_noalloc = 1; }
+        ~Guard() { t_noalloc = 0; }
+    } guard;
+    const int dir = op == NFFT_OP_DFT ? -1 : 1;
+    const hipError_t e = dft_nd(*plan, plan->ax, plan->dims, plan->d, plan->batch, in, out, dir, op == NFFT_OP_DFT,
+                                1.0, scr, (long long)(kStaticScratchBytes / cb), (hipStream_t)stream);
+    if (e != hipSuccess) {
+        set_error(std::string("execute_noalloc: ") + hipGetErrorString(e));
+        return NFFT_E_HIP;
+    }
+    return NFFT_OK;
 }
 
 int nfft_vjp(nfft_plan *plan, int op, const void *grad_out, void *grad_in, void *stream)
