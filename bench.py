@@ -7,9 +7,11 @@ in HBM before the timed region.  A step is one forward transform of the
 whole batch.  value = 5 N log2 N * batch / t (GFLOP/s).
 
 Also printed in the same JSON line:
-  roofline      the dominant kernel's algorithmic bytes per launch divided by
-                its average launch time, timed live with HIP events by the
-                library's kernel timer on the launch stream (second timed pass)
+  roofline      the algorithmic bytes of one step (one read and one write of
+                the batch) over the summed average durations of the kernels of
+                a step, timed live with HIP events by the library's kernel
+                timer on the launch stream (second timed pass); the headline
+                step is one launch of the XCD-affine four-step k_four_xcd
   cpu_baseline  the reference library compiled from its own header
                 (oracle/_ref, FP64, OpenMP over the batch, one aux per thread)
                 on a bounded sample of the same workload on this host
@@ -391,8 +393,9 @@ def transform_roofline(head, n, total, steps):
     read and one write of the batch) over the summed average durations of
     every kernel launched per step (HIP events on the launch stream), against
     the 8 TB/s HBM peak.  Per-kernel figures stay under "kernels"; "traffic"
-    is the HBM bytes per step from the committed PMC summary when it covers
-    every kernel of the step."""
+    is the bytes per step between the L2 caches and the memory side (memory-
+    side cache + HBM: an upper bound on HBM traffic) from the committed PMC
+    summary when it covers every kernel of the step."""
     kern = head["kernels"]
     alg = 2.0 * total * 8
     step_us = sum(v["avg_us"] * v["launches"] / steps for v in kern.values())
