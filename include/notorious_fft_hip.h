@@ -97,6 +97,21 @@ int nfft_execute_noalloc(nfft_plan *plan, int op, const void *in, void *out, voi
  *   dct4/dst4:  self-adjoint */
 int nfft_vjp(nfft_plan *plan, int op, const void *grad_out, void *grad_in, void *stream);
 
+/* Weighted spectral power of each transform of a DFT plan's batch and its
+ * gradient (the adjoint pipeline of loss = sum_k |W_k DFT(x)_k|^2):
+ *   loss[t] = sum_k |W_k Y_k|^2 with Y = DFT(x_t) (unnormalised), in FP64
+ *             (device array of `batch` doubles; NULL: not written);
+ *   grad    = 2 DFT^H (W^2 . Y), the convention of nfft_vjp
+ *             (g = dL/dRe x + i dL/dIm x), layout of x (grad == x allowed).
+ * w: prod(dims) real weights (the plan's precision), shared by the batch.
+ * For d >= 2 with a power-of-two axis 0 of at most 4096 points, the forward
+ * transform of axis 0, the weighting and the adjoint of axis 0 run as one
+ * fused pass; otherwise forward, elementwise weighting and adjoint run as
+ * separate passes.  No counterpart in the reference (whose gradient would be
+ * its invdft of the weighted spectrum, src/07_legacy.h:146-152). */
+int nfft_dft_weighted_power(nfft_plan *plan, const void *x, const void *w, void *grad, double *loss,
+                            void *stream);
+
 /* scratch bytes held by the plan in HBM */
 long long nfft_plan_workspace_bytes(const nfft_plan *plan);
 

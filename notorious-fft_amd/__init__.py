@@ -81,6 +81,8 @@ def core():
         lib.nfft_execute.argtypes = [_vp, _i, _vp, _vp, _vp]
         lib.nfft_vjp.restype = _i
         lib.nfft_vjp.argtypes = [_vp, _i, _vp, _vp, _vp]
+        lib.nfft_dft_weighted_power.restype = _i
+        lib.nfft_dft_weighted_power.argtypes = [_vp, _vp, _vp, _vp, _vp, _vp]
         lib.nfft_execute_noalloc.restype = _i
         lib.nfft_execute_noalloc.argtypes = [_vp, _i, _vp, _vp, _vp]
         lib.nfft_plan_workspace_bytes.restype = _ll
@@ -176,6 +178,12 @@ class Plan:
     def execute_noalloc(self, op, src_ptr, dst_ptr, stream=0):
         """nfft_execute_noalloc: the library's static scratch instead of the plan's."""
         check(core().nfft_execute_noalloc(self.handle, OP[op], src_ptr, dst_ptr, stream), f"execute_noalloc {op}")
+
+    def weighted_power(self, x_ptr, w_ptr, grad_ptr, loss_ptr, stream=0):
+        """nfft_dft_weighted_power: loss[t] = sum |W DFT(x_t)|^2 (FP64 device array) and
+        grad = 2 DFT^H (W^2 DFT(x))."""
+        check(core().nfft_dft_weighted_power(self.handle, x_ptr, w_ptr, grad_ptr, loss_ptr, stream),
+              "dft_weighted_power")
 
     def vjp(self, op, g_ptr, out_ptr, stream=0):
         check(core().nfft_vjp(self.handle, OP[op], g_ptr, out_ptr, stream), f"vjp {op}")

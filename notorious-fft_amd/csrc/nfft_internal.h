@@ -106,6 +106,26 @@ bool persist_find(int prec, long long n, int variant, PersistCfg *cfg);
 hipError_t launch_four_persist(const PersistCfg &c, int dir, const PersistArgs &a, int grid, hipStream_t s);
 hipError_t launch_persist_reset(unsigned *ctr, long long words, hipStream_t s);
 
+// weighted spectral power (nfft_k_wpow.hip): fused column pass of axis 0
+// (length L, ncols columns of stride ncols), its tiles per transform, and the
+// elementwise / reduction passes
+struct WpowArgs {
+    void *buf;                 // in place: batch x prod(dims) complex
+    const void *w;             // weights W, prod(dims) reals (shared by the batch)
+    double *partial;           // one FP64 partial sum per workgroup
+    int ncols;                 // S: columns of axis 0 (prod(dims[1:]))
+    long long dist;            // elements per transform (L * S)
+    long long ntrans;          // batch
+    const void *tw;            // FP64 stage table of L
+};
+bool col_wpow_supported(int prec, int L);
+long long col_wpow_tiles(int prec, int L, int ncols);
+hipError_t launch_col_wpow(int prec, int L, const WpowArgs &a, hipStream_t s);
+long long wpow_mask_chunks(long long P);
+hipError_t launch_wpow_mask(int prec, void *buf, const void *w, double *partial, long long P, long long ntrans,
+                            hipStream_t s);
+hipError_t launch_wpow_reduce(const double *partial, double *loss, long long per, long long ntrans, hipStream_t s);
+
 // XCD-affine FP32 four-step (nfft_k_xcd.hip): true if length n has an
 // instance (N1 x 1024); n1 and the resident workgroups per CU are returned.
 bool xcd_supported(long long n, int *n1, int *per_cu);

@@ -1204,6 +1204,8 @@ void nfft_plan::release()
     for (auto &t : tax) t.release();
     if (work) (void)hipFree(work);
     work = nullptr;
+    if (wpow_part) (void)hipFree(wpow_part);
+    wpow_part = nullptr;
 }
 
 // complex N-D over dims[0..nd-1] of `lines` stacked arrays, in place on buf
@@ -1368,6 +1370,75 @@ int nfft_plan::execute(int op, const void *in, void *out, hipStream_t s, int vjp
         return NFFT_E_HIP;
     }
     (void)rb;
+    return NFFT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Weighted spectral power L = sum |W Y|^2 and its gradient 2 DFT^H (W^2 Y).
+// Where axis 0 has a single-pass column kernel (d >= 2), the forward
+// transform of axis 0, the weighting and the adjoint of axis 0 are one fused
+// pass (k_col_wpow) between the other axes' forward and adjoint passes;
+// otherwise forward transform, elementwise weighting, adjoint (NFFT_FUSE=0
+// forces the latter).
+// ---------------------------------------------------------------------------
+int nfft_plan::weighted_power(const void *x, const void *w, void *grad, double *loss, hipStream_t s)
+{
+    if (kind != NFFT_KIND_DFT) {
+        set_error("weighted power needs a DFT plan");
+        return NFFT_E_INVALID;
+    }
+    const size_t cb = cx_bytes(prec);
+    const long long eng_elems = (long long)(work_bytes / cb);
+    const long long S = P / dims[0];
+    const bool fused = d >= 2 && env_int("NFFT_FUSE", 1) != 0 && ax[0].algo == ALG_LINE &&
+                       col_wpow_supported(prec, dims[0]) && S <= 0x7fffffffLL;
+    const long long per = fused ? col_wpow_tiles(prec, dims[0], (int)S) : wpow_mask_chunks(P);
+    if (per < 1) {
+        set_error("weighted power: no kernel for this shape");
+        return NFFT_E_UNSUPPORTED;
+    }
+    if (batch * per > wpow_cap) {
+        if (wpow_part) (void)hipFree(wpow_part);
+        wpow_part = nullptr;
+        wpow_cap = 0;
+        if (hipMalloc(&wpow_part, (size_t)(batch * per) * sizeof(double)) != hipSuccess) {
+            set_error("hipMalloc of the weighted-power partial sums failed");
+            return NFFT_E_ALLOC;
+        }
+        wpow_cap = batch * per;
+    }
+    hipError_t e = hipSuccess;
+    constexpr long long kGroup = 65535;  // transforms per weighting launch (grid.y)
+    if (fused) {
+        // forward along axes d-1..1 of the batch x N0 sub-arrays
+        const std::vector<C2C> sub(ax.begin() + 1, ax.end());
+        const std::vector<int> subd(dims.begin() + 1, dims.end());
+        e = dft_nd(*this, sub, subd, d - 1, batch * dims[0], x, grad, -1, true, 1.0, work, eng_elems, s);
+        for (long long t0 = 0; t0 < batch && e == hipSuccess; t0 += kGroup) {
+            WpowArgs a{};
+            a.buf = (char *)grad + (size_t)t0 * P * cb;
+            a.w = w;
+            a.partial = wpow_part + t0 * per;
+            a.ncols = (int)S;
+            a.dist = P;
+            a.ntrans = std::min(kGroup, batch - t0);
+            a.tw = ax[0].tw_a;
+            e = launch_col_wpow(prec, dims[0], a, s);
+        }
+        // adjoint (unnormalised inverse) along axes d-1..1
+        if (e == hipSuccess) e = dft_nd(*this, sub, subd, d - 1, batch * dims[0], grad, grad, 1, true, 1.0, work, eng_elems, s);
+    } else {
+        e = dft_nd(*this, ax, dims, d, batch, x, grad, -1, true, 1.0, work, eng_elems, s);
+        for (long long t0 = 0; t0 < batch && e == hipSuccess; t0 += kGroup)
+            e = launch_wpow_mask(prec, (char *)grad + (size_t)t0 * P * cb, w, wpow_part + t0 * per, P,
+                                 std::min(kGroup, batch - t0), s);
+        if (e == hipSuccess) e = dft_nd(*this, ax, dims, d, batch, grad, grad, 1, true, 1.0, work, eng_elems, s);
+    }
+    if (e == hipSuccess && loss) e = launch_wpow_reduce(wpow_part, loss, per, batch, s);
+    if (e != hipSuccess) {
+        set_error(std::string("weighted power: ") + hipGetErrorString(e));
+        return NFFT_E_HIP;
+    }
     return NFFT_OK;
 }
 
@@ -1539,6 +1610,16 @@ int nfft_execute_noalloc(nfft_plan *plan, int op, const void *in, void *out, voi
         return NFFT_E_HIP;
     }
     return NFFT_OK;
+}
+
+int nfft_dft_weighted_power(nfft_plan *plan, const void *x, const void *w, void *grad, double *loss, void *stream)
+{
+    clear_error();
+    if (!plan || !x || !w || !grad) {
+        set_error("NULL plan or buffer");
+        return NFFT_E_INVALID;
+    }
+    return plan->weighted_power(x, w, grad, loss, (hipStream_t)stream);
 }
 
 int nfft_vjp(nfft_plan *plan, int op, const void *grad_out, void *grad_in, void *stream)

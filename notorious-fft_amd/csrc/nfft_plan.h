@@ -60,9 +60,13 @@ struct nfft_plan {
     void *work = nullptr;
     size_t work_bytes = 0;
     std::string desc;
+    double *wpow_part = nullptr;      // loss partial sums of weighted_power (grown on demand)
+    long long wpow_cap = 0;
 
     bool init(int kind, int prec, int d, const int *dims, long long batch, int norm);
     void release();
     // vjp_mode != 0 runs the vector-Jacobian product of `op`
     int execute(int op, const void *in, void *out, hipStream_t s, int vjp_mode);
+    // loss[t] = sum |W Y_t|^2 (Y_t = DFT(x_t)) and grad = 2 DFT^H (W^2 Y)
+    int weighted_power(const void *x, const void *w, void *grad, double *loss, hipStream_t s);
 };
