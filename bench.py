@@ -139,11 +139,118 @@ def timed(torch, fn, steps=5, warmup=2):
     return (time.perf_counter() - t0) / steps
 
 
-def config_extras(pkg, torch, only=None):
+def ref_lib():
+    """(ctypes wrapper of the reference built with its Release flags, name), or
+    (None, reason) where no build for this host exists."""
+    import oracle_lib as ol
+    flags = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
+    name = "libnfft_ref_fast64.so" if ("avx512f" in flags and "avx512dq" in flags) else "libnfft_ref_fast64_avx2.so"
+    try:
+        return ol.Ref(name), name
+    except (FileNotFoundError, OSError):
+        return None, "no reference build"
+
+
+def cpu_config(op, dims, batch_full, sample, flops_per, post=None):
+    """The reference (FP64, Release flags, OpenMP over the sampled batch with
+    one aux per thread; a single N-D transform runs on one thread, as the
+    reference does) on `sample` transforms of a configuration: GFLOP/s at the
+    configuration's nominal flops and the estimated time of the full batch.
+    post(ref) adds work timed with it (C5: mask + inverse)."""
+    import oracle_lib as ol
+    ref, name = ref_lib()
+    if ref is None:
+        return None
+    cores = min(16, os.cpu_count() or 1)
+    nin, _ = ol.out_shape(op, dims)
+    x = ol.uniform(0, nin * sample)
+    y, t = ref.run_timed(op, dims, x, sample, cores)
+    if post is not None:
+        t += post(ref, y, cores)
+    return {"value": round(flops_per * sample / t / 1e9, 3), "unit": "GFLOP/s",
+            "cores": cores if sample > 1 else 1, "kind": "reference",
+            "sample": f"{sample} of {batch_full} transforms, {name}, {t:.2f} s in the transform loop",
+            "est_full_ms": round(1e3 * t * batch_full / sample, 2)}
+
+
+def gauss_mask(dims):
+    """W[k] = exp(-sum_a d(k_a)^2 / (2 sigma_a^2)), d(k) = min(k, N - k), sigma_a = N_a / 16."""
+    import numpy as np
+    w = np.ones(dims)
+    for a, n in enumerate(dims):
+        k = np.arange(n)
+        d = np.minimum(k, n - k).astype(np.float64)
+        sh = [1] * len(dims)
+        sh[a] = n
+        w = w * np.exp(-(d ** 2) / (2.0 * (n / 16.0) ** 2)).reshape(sh)
+    return w
+
+
+def masked_power(pkg, torch, out, wanted, with_cpu):
+    """Loss sum |W DFT2(x)|^2 and its gradient 2 DFT2^H(W^2 DFT2(x)), 2-D C2C
+    2048^2 x 16 with the Gaussian-bump mask: FP32 fused (axis-0 forward,
+    weighting and adjoint in one pass), FP32 unfused, the plain FP64 GPU path
+    (unfused), and the reference on the CPU (dft, weighting, invdft)."""
+    import numpy as np
+    dims, batch = [2048, 2048], 16
+    p = 2048 * 2048
+    flops = 2 * 5 * p * 22  # forward + adjoint
+    dev = torch.device("cuda:0")
+    for name, prec, fuse in (("c5_loss_grad_f32_2048^2_x16_fused", "f32", "1"),
+                             ("c5_loss_grad_f32_2048^2_x16_unfused", "f32", "0"),
+                             ("c5_loss_grad_f64_2048^2_x16_unfused", "f64", "0")):
+        if not wanted(name):
+            continue
+        os.environ["NFFT_FUSE"] = fuse
+        dt = torch.float32 if prec == "f32" else torch.float64
+        x = torch.empty(2 * p * batch, dtype=dt, device=dev)
+        g = torch.empty_like(x)
+        loss = torch.empty(batch, dtype=torch.float64, device=dev)
+        pkg.fill_uniform(x.data_ptr(), x.numel(), 0, 0, prec)
+        w = torch.from_numpy(gauss_mask(dims).reshape(-1).astype(np.float32 if prec == "f32" else np.float64)).to(dev)
+        pl = pkg.Plan("dft", prec, dims, batch)
+        t = timed(torch, lambda: pl.weighted_power(x.data_ptr(), w.data_ptr(), g.data_ptr(), loss.data_ptr()),
+                  steps=20, warmup=10)
+        eb = 8 if prec == "f32" else 16
+        alg = 2 * p * batch * eb + p * eb // 2  # x read, grad written, W read once
+        out[name] = {"ms": round(1e3 * t, 4), "gflops": round(flops * batch / t / 1e9, 1),
+                     "alg_gbs": round(alg / t / 1e9, 1), "hbm_frac": round(alg / t / 1e9 / HBM_PEAK_GBS, 4),
+                     "loss_0": float(loss[0].item()), "plan": pl.describe()}
+        log(f"  {name}: {out[name]}")
+        pl.close()
+        del x, g, w
+        torch.cuda.empty_cache()
+    os.environ.pop("NFFT_FUSE", None)
+    name = "c5_loss_grad_f32_2048^2_x16_fused"
+    if with_cpu and name in out:
+        w2 = np.repeat(gauss_mask(dims).reshape(-1) ** 2, 2)
+
+        def post(ref, y, cores):
+            # weighting (numpy, timed) and the reference's invdft of the weighted spectra
+            t0 = time.perf_counter()
+            v = 2 * w2 * y.reshape(16, -1)
+            t1 = time.perf_counter()
+            _, ti = ref.run_timed("invdft", dims, v.reshape(-1), 16, cores)
+            return (t1 - t0) + ti
+        try:
+            c = cpu_config("dft", dims, batch, batch, flops, post=post)
+        except Exception as e:
+            c = {"error": str(e)}
+        if c is not None:
+            out[name]["cpu_baseline"] = c
+            if "value" in c:
+                out[name]["gpu_over_cpu"] = round(out[name]["gflops"] / c["value"], 1)
+            log(f"  {name} cpu: {c}")
+
+
+def config_extras(pkg, torch, only=None, with_cpu=True):
     """The other workloads of the configuration list, one timing each (FP32
     unless stated): R2C batches, 3D C2C 512^3, 2D R2C 4096^2, DCT-2/DST-2/DCT-4
-    rows, the ragged batch of 10,000 mixed sizes (FP32 and FP64) and the
-    adjoint pair (forward + vjp) of the 2D C2C and the DCT-2."""
+    rows, the ragged batch of 10,000 mixed sizes (FP32 and FP64), the adjoint
+    pair (forward + vjp) of the 2D C2C and the DCT-2, and the masked-power
+    loss + gradient of the 2D C2C (fused FP32, unfused FP32 and the plain FP64
+    path).  With with_cpu, each carries the reference's CPU time on a bounded
+    sample of the same workload ("cpu_baseline")."""
     import ctypes
     from ragged_sizes import ragged_sizes
 
@@ -152,6 +259,18 @@ def config_extras(pkg, torch, only=None):
 
     def wanted(name):
         return not only or any(name.startswith(o) for o in only)
+
+    def with_cpu_line(name, *args, **kw):
+        if with_cpu and name in out:
+            try:
+                c = cpu_config(*args, **kw)
+            except Exception as e:  # the baseline is a report, never a reason to fail the bench
+                c = {"error": str(e)}
+            if c is not None:
+                out[name]["cpu_baseline"] = c
+                if "value" in c and "gflops" in out[name]:
+                    out[name]["gpu_over_cpu"] = round(out[name]["gflops"] / c["value"], 1)
+                log(f"  {name} cpu: {c}")
 
     def plan_run(name, kind, prec, dims, batch, op, nin, nout, flops, vjp=False):
         if not wanted(name):
@@ -178,26 +297,49 @@ def config_extras(pkg, torch, only=None):
         torch.cuda.empty_cache()
 
     n = 1 << 16
-    plan_run("r2c_f32_N2^16_x4096", "realdft", "f32", [n], 4096, "realdft", n, n + 2,
-             2.5 * n * math.log2(n))
+    fl = 2.5 * n * math.log2(n)
+    plan_run("r2c_f32_N2^16_x4096", "realdft", "f32", [n], 4096, "realdft", n, n + 2, fl)
+    with_cpu_line("r2c_f32_N2^16_x4096", "realdft", [n], 4096, 4096, fl)
     n = 1 << 20
-    plan_run("r2c_f32_N2^20_x256", "realdft", "f32", [n], 256, "realdft", n, n + 2, 2.5 * n * math.log2(n))
+    fl = 2.5 * n * math.log2(n)
+    plan_run("r2c_f32_N2^20_x256", "realdft", "f32", [n], 256, "realdft", n, n + 2, fl)
+    with_cpu_line("r2c_f32_N2^20_x256", "realdft", [n], 256, 256, fl)
     p3 = 512 ** 3
     plan_run("c2c3d_f32_512^3", "dft", "f32", [512, 512, 512], 1, "dft", 2 * p3, 2 * p3, 5 * p3 * 27)
+    with_cpu_line("c2c3d_f32_512^3", "dft", [512, 512, 512], 1, 1, 5 * p3 * 27)
     p2 = 4096 * 4096
     plan_run("r2c2d_f32_4096^2", "realdft", "f32", [4096, 4096], 1, "realdft", p2, 4096 * 2049 * 2,
              2.5 * p2 * 24)
+    if with_cpu and "r2c2d_f32_4096^2" in out:
+        # the reference declares but does not implement multi-D realdft: the
+        # oracle's restatement (FFTW semantics, one thread) stands in
+        import oracle_lib as ol
+        xr = ol.gaussian_np(0, p2)
+        t0 = time.perf_counter()
+        ol.run_oracle("realdft", [4096, 4096], xr)
+        t = time.perf_counter() - t0
+        out["r2c2d_f32_4096^2"]["cpu_baseline"] = {
+            "value": round(2.5 * p2 * 24 / t / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+            "sample": f"the whole transform through the oracle's multi-D realdft, {t:.2f} s"}
+        out["r2c2d_f32_4096^2"]["gpu_over_cpu"] = round(
+            out["r2c2d_f32_4096^2"]["gflops"] / out["r2c2d_f32_4096^2"]["cpu_baseline"]["value"], 1)
     n = 1 << 14
+    fl = 2.5 * n * math.log2(n)
     for op in ("dct2", "dst2", "dct3", "dct4"):
-        plan_run(f"{op}_f32_N2^14_x8192", "t2t3" if op != "dct4" else "t4", "f32", [n], 8192, op, n, n,
-                 2.5 * n * math.log2(n))
+        plan_run(f"{op}_f32_N2^14_x8192", "t2t3" if op != "dct4" else "t4", "f32", [n], 8192, op, n, n, fl)
+        with_cpu_line(f"{op}_f32_N2^14_x8192", op, [n], 8192, 8192, fl)
     p2 = 2048 * 2048
     plan_run("adj_c2c2d_f32_2048^2_x16_fwd", "dft", "f32", [2048, 2048], 16, "dft", 2 * p2, 2 * p2, 5 * p2 * 22)
+    with_cpu_line("adj_c2c2d_f32_2048^2_x16_fwd", "dft", [2048, 2048], 16, 16, 5 * p2 * 22)
     plan_run("adj_c2c2d_f32_2048^2_x16_vjp", "dft", "f32", [2048, 2048], 16, "dft", 2 * p2, 2 * p2, 5 * p2 * 22,
              vjp=True)
+    with_cpu_line("adj_c2c2d_f32_2048^2_x16_vjp", "invdft", [2048, 2048], 16, 16, 5 * p2 * 22)
     n = 1 << 14
-    plan_run("adj_dct2_f32_N2^14_x1024_vjp", "t2t3", "f32", [n], 1024, "dct2", n, n, 2.5 * n * math.log2(n),
-             vjp=True)
+    fl = 2.5 * n * math.log2(n)
+    plan_run("adj_dct2_f32_N2^14_x1024_vjp", "t2t3", "f32", [n], 1024, "dct2", n, n, fl, vjp=True)
+    # the vjp of dct2 is the reference's dct3 with the DC input doubled
+    with_cpu_line("adj_dct2_f32_N2^14_x1024_vjp", "dct3", [n], 1024, 1024, fl)
+    masked_power(pkg, torch, out, wanted, with_cpu)
     # the drop-in C API on host buffers (staged through the aux's device
     # buffers, so PCIe is in the loop): configuration C1 (FP64 N = 4096,
     # dft then invdft, seed 0) and one large FP32 transform
@@ -225,6 +367,15 @@ def config_extras(pkg, torch, only=None):
         lib64.notorious_fft_free_aux(a)
         out["dropin_c1_f64_N4096_dft_invdft_host"] = {"us_per_round_trip": round(1e6 * t, 1),
                                                        "round_trip_rel_l2": float(f"{rt:.3e}")}
+        ref, rname = ref_lib() if with_cpu else (None, "")
+        if ref is not None:
+            # the reference itself on one thread: 1000 dft + 1000 invdft of N = 4096
+            xs = ol.uniform(0, 2 * n * 1000)
+            ys, t_f = ref.run_timed("dft", [n], xs, 1000, 1)
+            _, t_i = ref.run_timed("invdft", [n], ys, 1000, 1)
+            out["dropin_c1_f64_N4096_dft_invdft_host"]["cpu_baseline"] = {
+                "us_per_round_trip": round(1e6 * (t_f + t_i) / 1000, 2), "cores": 1, "kind": "reference",
+                "sample": f"1000 dft + 1000 invdft of N = 4096 on one thread, {rname}"}
         log(f"  dropin C1: {out['dropin_c1_f64_N4096_dft_invdft_host']}")
         lib32 = pkg.dropin("f32")
         n = 1 << 22
@@ -241,11 +392,14 @@ def config_extras(pkg, torch, only=None):
         out["dropin_f32_N2^22_host"] = {"ms": round(1e3 * t, 3), "pcie_inclusive_gbs": round(2 * n * 8 / t / 1e9, 2)}
         log(f"  dropin host 2^22: {out['dropin_f32_N2^22_host']}")
 
-    # 7-smooth lengths (mixed radix) beside the power-of-two sweep
+    # 7-smooth lengths (mixed radix) beside the power-of-two sweep; the
+    # reference runs them through Bluestein
     for prec, n in (("f32", 100), ("f64", 100), ("f64", 10000), ("f32", 10000)):
         batch = (1 << 27) // n
         plan_run(f"mixed_radix_{prec}_N{n}_x{batch}", "dft", prec, [n], batch, "dft", 2 * n, 2 * n,
                  5 * n * math.log2(n))
+        with_cpu_line(f"mixed_radix_{prec}_N{n}_x{batch}", "dft", [n], batch, 262144 if n == 100 else 4096,
+                      5 * n * math.log2(n))
     if not wanted("ragged"):
         return out
     sizes = ragged_sizes(10000)
@@ -487,7 +641,7 @@ def main():
 
     extras = {}
     if not args.no_sweep or only:
-        extras = config_extras(pkg, torch, only)
+        extras = config_extras(pkg, torch, only, with_cpu=not args.no_cpu)
 
     cpu = None
     if not args.no_cpu:
