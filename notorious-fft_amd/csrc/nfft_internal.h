@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -173,6 +174,15 @@ hipError_t launch_transpose(const void *in, void *out, long long batch, long lon
                             hipStream_t s);
 // elementwise complex ops for the adjoint / loss helpers
 hipError_t launch_scale_real(void *buf, long long count, int prec, double s, hipStream_t st);
+// six-step pieces: out[b][c][r] = in[b][r][c] * w_N^{dir r c} * scale (tw: the
+// two-level FP64 table of N, rows * cols <= N)
+hipError_t launch_tw_transpose(int prec, const void *in, void *out, long long batch, long long rows, long long cols,
+                               const Tw2 &tw, int dir, double scale, hipStream_t s);
+// Bluestein through whole-length engines: stage 0 z = pad(chirp * conj?(x))
+// (x strided by dist), 1 z *= H, 2 y = conj?(chirp * z[0, n)) * scale (y
+// strided by dist); z holds ntrans x m points
+hipError_t launch_blue_ew(int prec, int stage, const void *src, void *dst, const void *table, long long n, long long m,
+                          long long dist, long long ntrans, int conj, double scale, hipStream_t s);
 
 // ---------------------------------------------------------------------------
 // per-device table cache (FP64-generated twiddles, shared by all plans)
@@ -188,7 +198,14 @@ const void *chirp_table(int prec, long long n, const double2 **chirp64);
 // ---------------------------------------------------------------------------
 // 1D complex engine over contiguous lines
 // ---------------------------------------------------------------------------
-enum C2CAlgo { ALG_LINE = 0, ALG_FOUR = 1, ALG_BLUE1 = 2, ALG_BLUE3 = 3, ALG_N1 = 4, ALG_SMOOTH = 5, ALG_SMOOTH4 = 6 };
+enum C2CAlgo {
+    ALG_LINE = 0, ALG_FOUR = 1, ALG_BLUE1 = 2, ALG_BLUE3 = 3, ALG_N1 = 4, ALG_SMOOTH = 5, ALG_SMOOTH4 = 6,
+    ALG_BIG = 7,     // power of two beyond the four-step: N1 x N2 over two sub-engines (six-step)
+    ALG_BLUEBIG = 8  // Bluestein with M beyond the three-pass form: M-point sub-engine
+};
+// Longest length a C2C engine accepts (2^30: offsets and exponents stay
+// within the 32-bit twiddle exponent and the int size of the C API).
+constexpr long long kMaxLen = 1LL << 30;
 
 struct C2C {
     int n = 0;
@@ -210,6 +227,10 @@ struct C2C {
     SmoothPlan sp2{};                 // ALG_SMOOTH4: N2
     int sgroup = 1;                   // ALG_SMOOTH: lines per workgroup; ALG_SMOOTH4: columns per tile
     int sgroup2 = 1;                  // ALG_SMOOTH4: rows per tile
+    // ALG_BIG: sub1 (length n1) and sub2 (n2); ALG_BLUEBIG: sub1 (length m).
+    // Shared, so that copies of an engine (axis lists) use the same tables;
+    // release() of the owner releases them.
+    std::shared_ptr<C2C> sub1, sub2;
     // ALG_FOUR in one persistent launch (k_four_persist) when pctr is set
     PersistCfg pcfg{};
     unsigned *pctr = nullptr;         // owned: counter block for up to pcap transforms

@@ -329,6 +329,128 @@ hipError_t launch_transpose(const void *in, void *out, long long batch, long lon
     return hipSuccess;
 }
 
+// ---------------------------------------------------------------------------
+// Six-step pieces for lengths beyond the four-step (N = N1 x N2, both
+// transformed by ordinary engines): out[b][c][r] = in[b][r][c] * w_N^{dir r c}
+// * scale through a 32 x 33 LDS tile; the twiddle from the FP64 two-level
+// table (exponent r c < N, exact), the product in FP64.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_tw_transpose(const Cx<T> *in, Cx<T> *out, long long rows, long long cols, Tw2 tw, int dir,
+                               double scale)
+{
+    __shared__ Cx<T> tile[32][33];
+    const long long b = blockIdx.z;
+    const long long r0 = (long long)blockIdx.y * 32, c0 = (long long)blockIdx.x * 32;
+    const Cx<T> *src = in + b * rows * cols;
+    Cx<T> *dst = out + b * rows * cols;
+    const int tx = threadIdx.x, ty = threadIdx.y;  // 32 x 8
+    for (int k = ty; k < 32; k += 8) {
+        const long long r = r0 + k, c = c0 + tx;
+        if (r < rows && c < cols) {
+            const Cx<T> v = src[r * cols + c];
+            double2 w = tw2_get(tw, (uint32_t)(r * c));
+            w.x *= scale;
+            w.y *= dir > 0 ? -scale : scale;
+            tile[k][tx] = cmk<Cx<T>>((T)((double)v.x * w.x - (double)v.y * w.y), (T)((double)v.x * w.y + (double)v.y * w.x));
+        }
+    }
+    __syncthreads();
+    for (int k = ty; k < 32; k += 8) {
+        const long long c = c0 + k, r = r0 + tx;
+        if (r < rows && c < cols) dst[c * rows + r] = tile[tx][k];
+    }
+}
+
+hipError_t launch_tw_transpose(int prec, const void *in, void *out, long long batch, long long rows, long long cols,
+                               const Tw2 &tw, int dir, double scale, hipStream_t s)
+{
+    if (batch <= 0 || rows <= 0 || cols <= 0) return hipSuccess;
+    if ((cols + 31) / 32 > 0x7fffffffLL || (rows + 31) / 32 > 65535 || rows * cols > (1LL << 32))
+        return hipErrorInvalidValue;
+    const size_t esz = prec == F32 ? 8 : 16;
+    for (long long b0 = 0; b0 < batch; b0 += 65535) {
+        const long long nb = batch - b0 < 65535 ? batch - b0 : 65535;
+        const size_t off = (size_t)b0 * rows * cols * esz;
+        dim3 g((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32), (unsigned)nb), bl(32, 8);
+        if (prec == F32)
+            hipLaunchKernelGGL((k_tw_transpose<float>), g, bl, 0, s, (const float2 *)((const char *)in + off),
+                               (float2 *)((char *)out + off), rows, cols, tw, dir, scale);
+        else
+            hipLaunchKernelGGL((k_tw_transpose<double>), g, bl, 0, s, (const double2 *)((const char *)in + off),
+                               (double2 *)((char *)out + off), rows, cols, tw, dir, scale);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// Bluestein through whole-length engines (lengths beyond the three-pass
+// form): Z = pad(chirp * conj?(x)) over M points; Z *= H; y = conj?(chirp *
+// Z[0, n)) * scale.  blockIdx.y = transform.
+template <typename T>
+__global__ void k_blue_pre(const Cx<T> *x, Cx<T> *z, const Cx<T> *chirp, long long n, long long m, long long in_dist,
+                           int conj)
+{
+    const long long t = blockIdx.y;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (long long)gridDim.x * blockDim.x) {
+        Cx<T> v = czero<T>();
+        if (k < n) {
+            v = x[t * in_dist + k];
+            if (conj) v = cconj(v);
+            v = cmul(v, chirp[k]);
+        }
+        z[t * m + k] = v;
+    }
+}
+template <typename T> __global__ void k_blue_mulH(Cx<T> *z, const Cx<T> *H, long long m)
+{
+    const long long t = blockIdx.y;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < m; k += (long long)gridDim.x * blockDim.x)
+        z[t * m + k] = cmul(z[t * m + k], H[k]);
+}
+template <typename T>
+__global__ void k_blue_post(const Cx<T> *z, Cx<T> *y, const Cx<T> *chirp, long long n, long long m, long long out_dist,
+                            int conj, double scale)
+{
+    const long long t = blockIdx.y;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (long long)gridDim.x * blockDim.x) {
+        Cx<T> v = cmul(z[t * m + k], chirp[k]);
+        if (conj) v = cconj(v);
+        y[t * out_dist + k] = cscale(v, (T)scale);
+    }
+}
+
+hipError_t launch_blue_ew(int prec, int stage, const void *src, void *dst, const void *table, long long n, long long m,
+                          long long dist, long long ntrans, int conj, double scale, hipStream_t s)
+{
+    if (ntrans <= 0) return hipSuccess;
+    const long long len = stage == 2 ? n : m;
+    for (long long t0 = 0; t0 < ntrans; t0 += 65535) {
+        const long long nt = ntrans - t0 < 65535 ? ntrans - t0 : 65535;
+        const dim3 g((unsigned)std::min<long long>(1024, (len + 255) / 256), (unsigned)nt), b(256);
+        const size_t cb = prec == F32 ? 8 : 16;
+        const char *sp = (const char *)src;
+        char *dp = (char *)dst;
+        // offsets of this slice of transforms: src / dst strides by stage
+        const long long sstride = stage == 0 ? dist : m, dstride = stage == 2 ? dist : m;
+        sp += (size_t)t0 * sstride * cb;
+        dp += (size_t)t0 * dstride * cb;
+        if (prec == F32) {
+            if (stage == 0) hipLaunchKernelGGL(k_blue_pre<float>, g, b, 0, s, (const float2 *)sp, (float2 *)dp, (const float2 *)table, n, m, dist, conj);
+            else if (stage == 1) hipLaunchKernelGGL(k_blue_mulH<float>, g, b, 0, s, (float2 *)dp, (const float2 *)table, m);
+            else hipLaunchKernelGGL(k_blue_post<float>, g, b, 0, s, (const float2 *)sp, (float2 *)dp, (const float2 *)table, n, m, dist, conj, scale);
+        } else {
+            if (stage == 0) hipLaunchKernelGGL(k_blue_pre<double>, g, b, 0, s, (const double2 *)sp, (double2 *)dp, (const double2 *)table, n, m, dist, conj);
+            else if (stage == 1) hipLaunchKernelGGL(k_blue_mulH<double>, g, b, 0, s, (double2 *)dp, (const double2 *)table, m);
+            else hipLaunchKernelGGL(k_blue_post<double>, g, b, 0, s, (const double2 *)sp, (double2 *)dp, (const double2 *)table, n, m, dist, conj, scale);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 __global__ void k_scale_real(void *buf, long long count, int prec, double sc)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;

@@ -439,6 +439,24 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
         tw_a = stage_table(prec, L);
         return tw_a != nullptr;
     }
+    if (n > kMaxLen) {
+        set_error("transform length " + std::to_string(n) + " exceeds the library's limit of 2^30");
+        return false;
+    }
+    if (is_pow2(n) && (long long)n > (long long)col_max(prec) * rowT_max(prec)) {
+        // beyond the four-step: N = N1 x N2 over two engines (six-step:
+        // transpose, N1-point lines, twiddle + transpose, N2-point lines,
+        // transpose), N1 >= N2
+        const int lg = ilog2ll(n);
+        n2 = 1 << (lg / 2);
+        n1 = n / n2;
+        algo = ALG_BIG;
+        sub1 = std::make_shared<C2C>();
+        sub2 = std::make_shared<C2C>();
+        if (!sub1->init(n1, prec) || !sub2->init(n2, prec) || !tw2_table(n, &tw2)) return false;
+        scratch_per_line = 2LL * n + std::max(sub1->scratch_per_line * n2, sub2->scratch_per_line * n1);
+        return true;
+    }
     if (is_pow2(n)) {
         algo = ALG_FOUR;
         scratch_per_line = n;
@@ -540,10 +558,20 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
         const int lg2 = std::min(lg / 2, ilog2ll(bluerow_max(prec)));
         m2 = 1 << lg2;
         m1 = m / m2;
-        if (m1 > col_max(prec)) {
+    }
+    if (m > line_max(prec) && m1 > col_max(prec)) {
+        // M beyond the three-pass form: chirp, FFT_M, x H, IFFT_M, chirp as
+        // elementwise passes around an M-point engine
+        if (m > kMaxLen) {
             set_error("Bluestein length " + std::to_string(n) + " exceeds the library's limit");
             return false;
         }
+        algo = ALG_BLUEBIG;
+        m1 = 0;
+        sub1 = std::make_shared<C2C>();
+        if (!sub1->init(m, prec)) return false;
+        scratch_per_line = m + sub1->scratch_per_line;
+    } else if (m > line_max(prec)) {
         algo = ALG_BLUE3;
         tw_a = stage_table(prec, m1);
         tw_b = stage_table(prec, m2);
@@ -567,6 +595,7 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
     (void)hipFree(hbuf);
     (void)hipFree(Hbuf);
     if (sbuf) (void)hipFree(sbuf);
+    h64.release();
     if (!ok) {
         if (H) (void)hipFree(H);
         H = nullptr;
@@ -582,6 +611,10 @@ void C2C::release()
     pctr = nullptr;
     if (xctr) (void)hipFree(xctr);
     xctr = nullptr;
+    for (auto *sub : {&sub1, &sub2}) {
+        if (*sub) (*sub)->release();
+        sub->reset();
+    }
 }
 
 bool C2C::device_fault() const
@@ -620,6 +653,13 @@ std::string C2C::describe() const
     }
     case ALG_SMOOTH4: snprintf(b, sizeof b, "mixed-radix four-step %dx%d", n1, n2); break;
     case ALG_BLUE3: snprintf(b, sizeof b, "bluestein N=%d M=%d (%dx%d)", n, m, m1, m2); break;
+    case ALG_BIG:
+        snprintf(b, sizeof b, "six-step %dx%d [%s | %s]", n1, n2, sub1 ? sub1->describe().c_str() : "?",
+                 sub2 ? sub2->describe().c_str() : "?");
+        break;
+    case ALG_BLUEBIG:
+        snprintf(b, sizeof b, "bluestein N=%d M=%d [%s]", n, m, sub1 ? sub1->describe().c_str() : "?");
+        break;
     default: snprintf(b, sizeof b, "n=%d", n);
     }
     return b;
@@ -745,6 +785,55 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
                        row_pf};
             if (row_twin) r.twf = tw2f;
             e = launch_rowT(prec, n2, dir, r, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    case ALG_BIG: {
+        // per chunk of transforms: scratch [A: cnt n][B: cnt n][engine scratch]
+        if (scratch_lines < 1 || !scratch) return hipErrorInvalidValue;
+        const long long big = (long long)n;
+        for (long long l0 = 0; l0 < nlines; l0 += scratch_lines) {
+            const long long cnt = std::min(scratch_lines, nlines - l0);
+            char *A = (char *)scratch, *B = A + (size_t)cnt * big * cb, *E = B + (size_t)cnt * big * cb;
+            const long long e_elems = (scratch_lines - cnt) * scratch_per_line + cnt * (scratch_per_line - 2 * big);
+            hipError_t e = hipSuccess;
+            // in (n1 x n2 per transform) -> A (n2 x n1)
+            for (long long t = 0; t < cnt && e == hipSuccess; t += (in_dist == big ? cnt : 1))
+                e = launch_transpose((const char *)in + (size_t)(l0 + t) * in_dist * cb, A + (size_t)t * big * cb,
+                                     in_dist == big ? cnt : 1, n1, n2, (int)cb, s);
+            if (e != hipSuccess) return e;
+            // n1-point lines, then the twiddle w_N^{r c} with the transpose to (n1 x n2)
+            const long long f1 = sub1->scratch_per_line ? e_elems / sub1->scratch_per_line : 0;
+            e = sub1->run(A, A, cnt * n2, n1, n1, dir, 1.0, E, f1, s);
+            if (e == hipSuccess) e = launch_tw_transpose(prec, A, B, cnt, n2, n1, tw2, dir, scale, s);
+            const long long f2 = sub2->scratch_per_line ? e_elems / sub2->scratch_per_line : 0;
+            if (e == hipSuccess) e = sub2->run(B, B, cnt * n1, n2, n2, dir, 1.0, E, f2, s);
+            // (n1 x n2) -> out (n2 x n1): point k1 + n1 k2
+            for (long long t = 0; t < cnt && e == hipSuccess; t += (out_dist == big ? cnt : 1))
+                e = launch_transpose(B + (size_t)t * big * cb, (char *)out + (size_t)(l0 + t) * out_dist * cb,
+                                     out_dist == big ? cnt : 1, n1, n2, (int)cb, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
+    }
+    case ALG_BLUEBIG: {
+        // per chunk: scratch [Z: cnt m][M-point engine scratch]
+        if (scratch_lines < 1 || !scratch) return hipErrorInvalidValue;
+        const int conj = dir > 0 ? 1 : 0;
+        for (long long l0 = 0; l0 < nlines; l0 += scratch_lines) {
+            const long long cnt = std::min(scratch_lines, nlines - l0);
+            char *Z = (char *)scratch, *E = Z + (size_t)cnt * m * cb;
+            const long long e_elems = (scratch_lines - cnt) * scratch_per_line + cnt * (scratch_per_line - m);
+            const long long fm = sub1->scratch_per_line ? e_elems / sub1->scratch_per_line : 0;
+            hipError_t e = launch_blue_ew(prec, 0, (const char *)in + (size_t)l0 * in_dist * cb, Z, chirp, n, m,
+                                          in_dist, cnt, conj, 1.0, s);
+            if (e == hipSuccess) e = sub1->run(Z, Z, cnt, m, m, -1, 1.0, E, fm, s);
+            if (e == hipSuccess) e = launch_blue_ew(prec, 1, nullptr, Z, H, n, m, m, cnt, 0, 1.0, s);
+            if (e == hipSuccess) e = sub1->run(Z, Z, cnt, m, m, 1, 1.0, E, fm, s);
+            if (e == hipSuccess)
+                e = launch_blue_ew(prec, 2, Z, (char *)out + (size_t)l0 * out_dist * cb, chirp, n, m, out_dist, cnt,
+                                   conj, scale, s);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
