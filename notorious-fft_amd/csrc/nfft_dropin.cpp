@@ -25,6 +25,11 @@
 #include "../../include/notorious_fft.h"
 #include "../../include/notorious_fft_hip.h"
 
+// the core library's device allocation (with its failure injection for tests)
+namespace nfft {
+hipError_t dev_malloc_raw(void **p, size_t bytes);
+}
+
 #ifdef NOTORIOUS_FFT_SINGLE
 #define NFFT_DROPIN_PREC NFFT_F32
 #else
@@ -128,7 +133,8 @@ void run(const notorious_fft_aux *a, int op, const void *x, void *y)
         if (im->dout) (void)hipFree(im->dout);
         im->din = im->dout = nullptr;
         im->stage_bytes = 0;
-        if (hipMalloc(&im->din, need) != hipSuccess || hipMalloc(&im->dout, need) != hipSuccess) return;
+        if (nfft::dev_malloc_raw(&im->din, need) != hipSuccess || nfft::dev_malloc_raw(&im->dout, need) != hipSuccess)
+            return;
         im->stage_bytes = need;
     }
     const void *src = x;

@@ -21,6 +21,17 @@ enum Prec { F32 = 0, F64 = 1 };
 inline size_t cx_bytes(int prec) { return prec == F32 ? 8 : 16; }
 inline size_t re_bytes(int prec) { return prec == F32 ? 4 : 8; }
 
+// Every device allocation of the library (nfft_plan.cpp).  Failure
+// injection for tests: NFFT_FAIL_ALLOC=k[:tag] fails the k-th allocation
+// counted from the first one that sees this exact value with
+// hipErrorOutOfMemory.  The count restarts only when an allocation sees a
+// different value, so a caller re-arming with the same k gives a new tag.
+hipError_t dev_malloc_raw(void **p, size_t bytes);
+// Device allocations that failed on this thread so far (constructors compare
+// it before and after to report NFFT_E_ALLOC rather than NFFT_E_HIP).
+long long alloc_failures();
+template <typename T> inline hipError_t dev_malloc(T **p, size_t bytes) { return dev_malloc_raw((void **)p, bytes); }
+
 // thread-local error message + status helpers (nfft_plan.cpp)
 void set_error(const std::string &msg);
 void clear_error();

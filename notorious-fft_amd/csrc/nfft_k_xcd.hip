@@ -34,6 +34,7 @@ void topo_probe(Topo &tp)
         return;
     const int g = std::max(64, 8 * ncu);
     int *d = nullptr;
+    // (not dev_malloc: a failed probe would switch the path off for the process)
     if (hipMalloc(&d, (size_t)g * sizeof(int)) != hipSuccess) return;
     std::vector<int> h((size_t)g, -1);
     hipStream_t st = nullptr;

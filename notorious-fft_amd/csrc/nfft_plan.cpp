@@ -29,6 +29,37 @@ namespace nfft {
 // errors
 // ---------------------------------------------------------------------------
 static thread_local std::string t_err;
+
+namespace {
+std::mutex g_alloc_mu;
+std::string g_fail_env;
+long long g_fail_at = 0, g_alloc_count = 0;
+}  // namespace
+
+static thread_local long long t_alloc_failures = 0;
+long long alloc_failures() { return t_alloc_failures; }
+
+hipError_t dev_malloc_raw(void **p, size_t bytes)
+{
+    {
+        std::lock_guard<std::mutex> lk(g_alloc_mu);
+        const char *v = getenv("NFFT_FAIL_ALLOC");
+        const std::string cur = v ? v : "";
+        if (cur != g_fail_env) {  // (re)armed: count from here; "k" or "k:tag"
+            g_fail_env = cur;
+            g_fail_at = cur.empty() ? 0 : atoll(cur.c_str());
+            g_alloc_count = 0;
+        }
+        if (g_fail_at > 0 && ++g_alloc_count == g_fail_at) {
+            *p = nullptr;
+            t_alloc_failures++;
+            return hipErrorOutOfMemory;
+        }
+    }
+    const hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) t_alloc_failures++;
+    return e;
+}
 // Set for the duration of an allocation-free execute: engines then use only
 // the scratch they are handed (no plan-owned counter blocks or slots).
 static thread_local int t_noalloc = 0;
@@ -136,7 +167,7 @@ template <typename Gen> static void *cache_get(const std::string &key, size_t by
     auto it = c.tabs.find(key);
     if (it != c.tabs.end()) return it->second;
     void *p = nullptr;
-    if (!hip_ok(hipMalloc(&p, bytes < 16 ? 16 : bytes), "hipMalloc(table)")) return nullptr;
+    if (!hip_ok(dev_malloc(&p, bytes < 16 ? 16 : bytes), "dev_malloc(table)")) return nullptr;
     if (!hip_ok(gen(p, st), "table generation") || !hip_ok(hipStreamSynchronize(st), "table generation")) {
         (void)hipFree(p);
         return nullptr;
@@ -307,8 +338,8 @@ static int tune_smooth(int n, int prec)
     const long long sp = std::max(cand[0].scratch_per_line, cand[1].scratch_per_line);
     float ms[2] = {1e30f, 1e30f};
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    bool ok = hipMalloc(&x, (size_t)lines * n * cb) == hipSuccess && hipMalloc(&y, (size_t)lines * n * cb) == hipSuccess &&
-              (sp == 0 || hipMalloc(&scr, (size_t)lines * sp * cb) == hipSuccess) && hipEventCreate(&e0) == hipSuccess &&
+    bool ok = dev_malloc(&x, (size_t)lines * n * cb) == hipSuccess && dev_malloc(&y, (size_t)lines * n * cb) == hipSuccess &&
+              (sp == 0 || dev_malloc(&scr, (size_t)lines * sp * cb) == hipSuccess) && hipEventCreate(&e0) == hipSuccess &&
               hipEventCreate(&e1) == hipSuccess &&
               launch_fill_uniform(x, 2 * lines * n, 5, 0, prec, st) == hipSuccess;
     for (int c = 0; c < 2 && ok; c++) {
@@ -363,7 +394,7 @@ static bool persist_setup(C2C &c)
         !hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "device attribute"))
         return false;
     const size_t bytes = (size_t)persist_ctr_words(kPersistCap) * sizeof(unsigned);
-    if (!hip_ok(hipMalloc(&c.pctr, bytes), "hipMalloc")) return false;
+    if (!hip_ok(dev_malloc(&c.pctr, bytes), "hipMalloc")) return false;
     if (!hip_ok(hipMemset(c.pctr, 0, bytes), "hipMemset")) {
         (void)hipFree(c.pctr);
         c.pctr = nullptr;
@@ -389,40 +420,38 @@ static const long long kXcdCap = 1LL << 12;
 // launches; 2^19 1.61 vs 1.61 ms and 2^21 1.80-1.82 vs 1.75 ms keep the two
 // launches).  NFFT_XCD=0 turns it off, NFFT_XCD=2 uses it for every length
 // that has an instance.
-static bool xcd_setup(C2C &c)
+// Returns 1 when the path is set up, 0 when it does not apply, -1 on an error
+// (error set).
+static int xcd_setup(C2C &c)
 {
     int n1 = 0, per_cu = 0;
     const int mode = env_int("NFFT_XCD", 1);
     if (c.prec != F32 || mode == 0 || (mode == 1 && c.n != (1 << 20)) || !xcd_supported(c.n, &n1, &per_cu))
-        return false;
+        return 0;
     int dev = 0, ncu = 0, khz = 0;
     if (!hip_ok(hipGetDevice(&dev), "hipGetDevice") ||
         !hip_ok(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "device attribute") ||
         !hip_ok(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev), "device attribute"))
-        return false;
+        return -1;
     const size_t bytes = (size_t)xcd_ctr_words(kXcdCap) * sizeof(unsigned);
-    if (!hip_ok(hipMalloc(&c.xctr, bytes), "hipMalloc")) return false;
+    if (!hip_ok(dev_malloc(&c.xctr, bytes), "hipMalloc")) return -1;
     if (!hip_ok(hipMemset(c.xctr, 0, bytes), "hipMemset")) {
         (void)hipFree(c.xctr);
         c.xctr = nullptr;
-        return false;
+        return -1;
     }
     c.xcap = kXcdCap;
     c.xgrid = std::max(1, ncu) * per_cu;
-    if (c.xgrid > kXcdMaxGrid || kXcdCap * std::max(XShape<512>::B, XShape<2048>::A) > kXcdMaxTiles) {
+    if (c.xgrid > kXcdMaxGrid || kXcdCap * std::max(XShape<512>::B, XShape<2048>::A) > kXcdMaxTiles ||
+        !xcd_topology(&c.xnq, c.xqmap)) {  // not applicable on this device
         (void)hipFree(c.xctr);
         c.xctr = nullptr;
-        return false;
-    }
-    if (!xcd_topology(&c.xnq, c.xqmap)) {
-        (void)hipFree(c.xctr);
-        c.xctr = nullptr;
-        return false;
+        return 0;
     }
     c.ptimeout = (long long)(kPersistTimeoutSec * 1000.0 * std::max(khz, 1000));
     c.n1 = n1;
     c.n2 = (int)(c.n / n1);
-    return true;
+    return 1;
 }
 
 bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
@@ -490,12 +519,16 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
             // FP32 1024-point rows take the inter-pass twiddle (column pass
             // 992 -> 910 us per 2^28 points; the prefetching row grid absorbs it)
             row_twin = prec == F32 && n2 == 1024 && env_int("NFFT_ROW_TWIDDLE", 1) != 0 ? 1 : 0;
-            if (row_twin && !tw2f_table(n, &tw2f)) row_twin = 0;
+            if (row_twin && !tw2f_table(n, &tw2f)) return false;
             // batches of at least 16 transforms: the XCD-affine single launch
             // (same N1 x 1024 split, inter-pass twiddle where row_twin puts it)
-            if (persist_mode != 0 && n2 == 1024 && xcd_setup(*this) && (n1 != n / 1024 || n2 != 1024)) {
-                set_error("internal: XCD four-step split");
-                return false;
+            if (persist_mode != 0 && n2 == 1024) {
+                const int x = xcd_setup(*this);
+                if (x < 0) return false;
+                if (x > 0 && (n1 != n / 1024 || n2 != 1024)) {
+                    set_error("internal: XCD four-step split");
+                    return false;
+                }
             }
         }
         tw_a = stage_table(prec, n1);
@@ -583,10 +616,10 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
     if (!h64.init(m, F64)) return false;
     hipStream_t st = setup_stream();
     void *hbuf = nullptr, *Hbuf = nullptr, *sbuf = nullptr;
-    bool ok = hip_ok(hipMalloc(&hbuf, (size_t)m * 16), "hipMalloc") &&
-              hip_ok(hipMalloc(&Hbuf, (size_t)m * 16), "hipMalloc") &&
-              hip_ok(hipMalloc(&H, (size_t)m * cx_bytes(prec)), "hipMalloc");
-    if (ok && h64.scratch_per_line) ok = hip_ok(hipMalloc(&sbuf, (size_t)h64.scratch_per_line * 16), "hipMalloc");
+    bool ok = hip_ok(dev_malloc(&hbuf, (size_t)m * 16), "hipMalloc") &&
+              hip_ok(dev_malloc(&Hbuf, (size_t)m * 16), "hipMalloc") &&
+              hip_ok(dev_malloc(&H, (size_t)m * cx_bytes(prec)), "hipMalloc");
+    if (ok && h64.scratch_per_line) ok = hip_ok(dev_malloc(&sbuf, (size_t)h64.scratch_per_line * 16), "hipMalloc");
     ok = ok && hip_ok(launch_blue_filter((double2 *)hbuf, chirp64, n, m, st), "filter") &&
          hip_ok(h64.run(hbuf, Hbuf, 1, m, m, -1, 1.0, sbuf, 1, st), "filter FFT") &&
          hip_ok(launch_blue_hprep((const double2 *)Hbuf, H, prec, m, algo == ALG_BLUE3 ? m1 : 0, m2, 1.0 / m, st),
@@ -1277,7 +1310,7 @@ bool nfft_plan::init(int kind_, int prec_, int d_, const int *dims_, long long b
     }
     desc = dsc;
     if (work_bytes) {
-        if (hipMalloc(&work, work_bytes) != hipSuccess) {
+        if (dev_malloc(&work, work_bytes) != hipSuccess) {
             set_error("hipMalloc of " + std::to_string(work_bytes) + " workspace bytes failed");
             work = nullptr;
             return false;
@@ -1490,7 +1523,7 @@ int nfft_plan::weighted_power(const void *x, const void *w, void *grad, double *
         if (wpow_part) (void)hipFree(wpow_part);
         wpow_part = nullptr;
         wpow_cap = 0;
-        if (hipMalloc(&wpow_part, (size_t)(batch * per) * sizeof(double)) != hipSuccess) {
+        if (dev_malloc(&wpow_part, (size_t)(batch * per) * sizeof(double)) != hipSuccess) {
             set_error("hipMalloc of the weighted-power partial sums failed");
             return NFFT_E_ALLOC;
         }
@@ -1636,13 +1669,14 @@ int nfft_plan_create(nfft_plan **plan, int kind, int prec, int d, const int *dim
         set_error("out of host memory");
         return NFFT_E_ALLOC;
     }
+    const long long fails0 = alloc_failures();
     if (!pl->init(kind, prec, d, dims, batch, norm)) {
         const std::string msg = last_error();
         pl->release();
         delete pl;
         set_error(msg);
         if (msg.find("limit") != std::string::npos) return NFFT_E_UNSUPPORTED;
-        if (msg.find("hipMalloc") != std::string::npos) return NFFT_E_ALLOC;
+        if (alloc_failures() > fails0 || msg.find("hipMalloc") != std::string::npos) return NFFT_E_ALLOC;
         return NFFT_E_HIP;
     }
     *plan = pl;

@@ -613,11 +613,16 @@ static void tune_store(const std::string &key, int val)
     fclose(f);
 }
 
+// Status of a failed table / engine setup: NFFT_E_ALLOC when a device
+// allocation failed since `fails0` (alloc_failures()), else NFFT_E_HIP.
+static int setup_fail(long long fails0) { return alloc_failures() > fails0 ? NFFT_E_ALLOC : NFFT_E_HIP; }
+
 extern "C" {
 
 int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes, int flags)
 {
     clear_error();
+    const long long fails0 = alloc_failures();
     if (!out || count < 1 || !sizes || (prec != NFFT_F32 && prec != NFFT_F64)) {
         set_error("invalid ragged arguments");
         return NFFT_E_INVALID;
@@ -632,6 +637,7 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
     std::map<int, int> size_tab;  // size -> table index
     std::map<int, int> size_kind; // size -> item type
     std::vector<RagTab> tabs;
+    // every size is checked before any device work
     for (int i = 0; i < count; i++) {
         const int n = sizes[i];
         if (n < 1) {
@@ -639,6 +645,16 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
             ragged_free(r);
             return NFFT_E_SIZE;
         }
+        long long m = 1;
+        while (m < 2LL * n - 1) m <<= 1;
+        if ((is_pow2(n) && n > 65536) || (!is_pow2(n) && m > 131072)) {
+            set_error("ragged batches support N <= 65536");
+            ragged_free(r);
+            return NFFT_E_UNSUPPORTED;
+        }
+    }
+    for (int i = 0; i < count; i++) {
+        const int n = sizes[i];
         offs[i] = r->total;
         r->total += n;
         if (size_tab.count(n)) continue;
@@ -660,7 +676,7 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
             size_kind[n] = RG_SMOOTH;
             t.sp1 = sp1;
             t.tw_a = phase_table(prec, n, 1, 0, n);
-            if (!t.tw_a) { ragged_free(r); return NFFT_E_HIP; }
+            if (!t.tw_a) { ragged_free(r); return setup_fail(fails0); }
         } else if (smooth_four) {
             size_kind[n] = RG_SMOOTH4;
             t.sp1 = sp1;
@@ -674,39 +690,39 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
             t.w2 = balanced(sp1.L, RG_SMOOTH_MAX / sp2.L);
             t.tw_a = phase_table(prec, sp1.L, 1, 0, sp1.L);
             t.tw_b = phase_table(prec, sp2.L, 1, 0, sp2.L);
-            if (!t.tw_a || !t.tw_b || !tw2_table(n, &t.tw2)) { ragged_free(r); return NFFT_E_HIP; }
+            if (!t.tw_a || !t.tw_b || !tw2_table(n, &t.tw2)) { ragged_free(r); return setup_fail(fails0); }
         } else if (p2 && n <= 4096) {
             size_kind[n] = RG_LINE;
             t.tw_a = stage_table(prec, n);
-            if (!t.tw_a) { ragged_free(r); return NFFT_E_HIP; }
+            if (!t.tw_a) { ragged_free(r); return setup_fail(fails0); }
         } else if (p2) {
             size_kind[n] = RG_FOUR;
             int n1, n2;
             four_split(n, &n1, &n2);
             t.tw_a = stage_table(prec, n1);
             t.tw_b = stage_table(prec, n2);
-            if (!t.tw_a || !t.tw_b || !tw2_table(n, &t.tw2)) { ragged_free(r); return NFFT_E_HIP; }
+            if (!t.tw_a || !t.tw_b || !tw2_table(n, &t.tw2)) { ragged_free(r); return setup_fail(fails0); }
         } else {
             // Bluestein: the engine builds chirp and H in the layout it uses
             // for M <= line limit (natural) or M1 x M2 (permuted); the ragged
             // kernel needs natural order for M <= 4096 and the [k1][k2] order
             // for its own M1 x M2 split, so H is rebuilt here when they differ.
-            if (!e.init(n, prec, 0)) { ragged_free(r); return NFFT_E_HIP; }
+            if (!e.init(n, prec, 0)) { ragged_free(r); return setup_fail(fails0); }
             t.chirp = e.chirp;
             size_kind[n] = m <= 4096 ? RG_BLUE1 : RG_BLUE3;
             if (m <= 4096) {
                 t.tw_a = stage_table(prec, (int)m);
-                if (e.algo != ALG_BLUE1) { set_error("unexpected Bluestein layout"); ragged_free(r); return NFFT_E_HIP; }
+                if (e.algo != ALG_BLUE1) { set_error("unexpected Bluestein layout"); ragged_free(r); return setup_fail(fails0); }
                 t.H = e.H;
             } else {
                 int m1, m2;
                 four_split((int)m, &m1, &m2);
                 t.tw_a = stage_table(prec, m1);
                 t.tw_b = stage_table(prec, m2);
-                if (!t.tw_a || !t.tw_b || !tw2_table(m, &t.tw2)) { ragged_free(r); return NFFT_E_HIP; }
+                if (!t.tw_a || !t.tw_b || !tw2_table(m, &t.tw2)) { ragged_free(r); return setup_fail(fails0); }
                 // permute H (natural or engine-permuted) into [k1][k2] for m1 x m2
                 void *Hn = nullptr;
-                if (hipMalloc(&Hn, (size_t)m * cb) != hipSuccess) { ragged_free(r); return NFFT_E_ALLOC; }
+                if (dev_malloc(&Hn, (size_t)m * cb) != hipSuccess) { ragged_free(r); return NFFT_E_ALLOC; }
                 r->owned.push_back(Hn);
                 // recompute H in FP64 from the definition with this split
                 C2C h64;
@@ -715,9 +731,9 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
                 chirp_table(prec, n, &chirp64);
                 hipStream_t st = nullptr;
                 (void)hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-                bool ok = h64.init((int)m, F64) && hipMalloc(&hb, (size_t)m * 16) == hipSuccess &&
-                          hipMalloc(&Hb, (size_t)m * 16) == hipSuccess;
-                if (ok && h64.scratch_per_line) ok = hipMalloc(&sb, (size_t)h64.scratch_per_line * 16) == hipSuccess;
+                bool ok = h64.init((int)m, F64) && dev_malloc(&hb, (size_t)m * 16) == hipSuccess &&
+                          dev_malloc(&Hb, (size_t)m * 16) == hipSuccess;
+                if (ok && h64.scratch_per_line) ok = dev_malloc(&sb, (size_t)h64.scratch_per_line * 16) == hipSuccess;
                 ok = ok && launch_blue_filter((double2 *)hb, chirp64, n, (int)m, st) == hipSuccess &&
                      h64.run(hb, Hb, 1, m, m, -1, 1.0, sb, 1, st) == hipSuccess &&
                      launch_blue_hprep((const double2 *)Hb, Hn, prec, (int)m, m1, m2, 1.0 / (double)m, st) ==
@@ -728,7 +744,7 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
                 if (sb) (void)hipFree(sb);
                 h64.release();
                 if (st) (void)hipStreamDestroy(st);
-                if (!ok) { set_error("ragged Bluestein table setup failed"); ragged_free(r); return NFFT_E_HIP; }
+                if (!ok) { set_error("ragged Bluestein table setup failed"); ragged_free(r); return setup_fail(fails0); }
                 t.H = Hn;
             }
             r->engines.push_back(e);  // keeps chirp/H alive
@@ -918,13 +934,13 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
     r->grid = std::max(1, std::min(r->nitems, ncu * per_cu));
     r->slot_elems = slot_elems;
     r->ctr_words = persist_ctr_words(0) + 3 * B;
-    bool ok = hipMalloc(&r->d_items, sorted.size() * sizeof(RagItem)) == hipSuccess &&
-              hipMalloc(&r->d_ids, ids.size() * sizeof(int)) == hipSuccess &&
-              hipMalloc(&r->d_offs, offs.size() * sizeof(long long)) == hipSuccess &&
-              hipMalloc(&r->d_tabs, tabs.size() * sizeof(RagTab)) == hipSuccess &&
-              hipMalloc((void **)&r->ctr, (size_t)r->ctr_words * sizeof(unsigned)) == hipSuccess &&
+    bool ok = dev_malloc(&r->d_items, sorted.size() * sizeof(RagItem)) == hipSuccess &&
+              dev_malloc(&r->d_ids, ids.size() * sizeof(int)) == hipSuccess &&
+              dev_malloc(&r->d_offs, offs.size() * sizeof(long long)) == hipSuccess &&
+              dev_malloc(&r->d_tabs, tabs.size() * sizeof(RagTab)) == hipSuccess &&
+              dev_malloc((void **)&r->ctr, (size_t)r->ctr_words * sizeof(unsigned)) == hipSuccess &&
               hipMemset(r->ctr, 0, (size_t)r->ctr_words * sizeof(unsigned)) == hipSuccess;
-    if (ok && B > 0) ok = hipMalloc(&r->scratch, (size_t)slots * slot_elems * cb) == hipSuccess;
+    if (ok && B > 0) ok = dev_malloc(&r->scratch, (size_t)slots * slot_elems * cb) == hipSuccess;
     ok = ok && hipMemcpy(r->d_items, sorted.data(), sorted.size() * sizeof(RagItem), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(r->d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice) == hipSuccess &&
          hipMemcpy(r->d_offs, offs.data(), offs.size() * sizeof(long long), hipMemcpyHostToDevice) == hipSuccess &&
@@ -939,7 +955,7 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
         // project-written autotuner: time the persistent launch for a few
         // workgroups-per-CU choices on a scratch batch and keep the fastest
         void *tin = nullptr, *tout = nullptr;
-        if (hipMalloc(&tin, (size_t)r->total * cb) == hipSuccess && hipMalloc(&tout, (size_t)r->total * cb) == hipSuccess) {
+        if (dev_malloc(&tin, (size_t)r->total * cb) == hipSuccess && dev_malloc(&tout, (size_t)r->total * cb) == hipSuccess) {
             (void)launch_fill_uniform(tin, 2 * r->total, 7, 0, prec, r->cap);
             hipEvent_t e0, e1;
             (void)hipEventCreate(&e0);
