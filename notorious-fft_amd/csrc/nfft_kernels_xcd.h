@@ -3,10 +3,15 @@
 //
 // The two-launch four-step (k_col, then k_rowT_pf) moves every point through
 // HBM four times: input read, intermediate write, intermediate read, output
-// write.  k_four_xcd keeps the intermediate of a transform on the XCD that
-// produces it: the column tiles and the row tiles of a transform are run by
-// workgroups of the same XCD, through the XCD's own L2 and the memory-side
-// cache behind it, so HBM sees the input read and the output write only.
+// write, in two launches.  k_four_xcd runs both passes in one launch and
+// keeps the intermediate of a transform on the XCD that produces it: the
+// column tiles and the row tiles of a transform are run by workgroups of the
+// same XCD, through the XCD's own L2 and the memory-side cache behind it.
+// An 8 MiB intermediate (N = 2^20) does not fit the 4 MB L2, so it still
+// crosses the fabric twice (counted by the L2's FETCH_SIZE / WRITE_SIZE);
+// what the single launch removes is the second pass's ramp and the
+// inter-launch gap, and the intermediate's slots (two per XCD, 128 MiB) stay
+// within the memory-side cache's capacity.
 //
 // Placement.  A workgroup reads the id of the XCD it runs on from the
 // hardware register XCC_ID and takes its work from that XCD's queue, so the
