@@ -60,16 +60,24 @@ template <int L> __host__ __device__ constexpr int line_lpb()
 
 // ---------------------------------------------------------------------------
 // Load / store of a tile of contiguous lines into reg[] (line-major mapping).
-// Direct global access when a line spans >= 64 threads (a full wavefront
-// reads 64 consecutive points), LDS-staged coalesced copy otherwise.
+// Direct global access when the NT threads of a line cover at least one
+// 128-byte segment per access (a wave then reads 64 / NT runs of whole
+// segments), LDS-staged coalesced copy for shorter lines.
 // ---------------------------------------------------------------------------
-template <typename T, int L, int LPB>
+// WIDE: direct access only for lines of a whole wave (NT >= 64), the rule
+// of the one-workgroup-per-tile row pass (see k_rowT).
+template <typename T, int L, bool WIDE = false> __host__ __device__ constexpr bool rows_direct()
+{
+    return WIDE ? Sched<L>::NT >= 64 : Sched<L>::NT * sizeof(Cx<T>) >= 128;
+}
+
+template <typename T, int L, int LPB, bool WIDE = false>
 __device__ __forceinline__ void tile_load(Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds, const Cx<T> *src, long long dist,
                                           int len, int nrows, int b, int j)
 {
     using S = Sched<L>;
     constexpr int NTH = S::NT * LPB;
-    if constexpr (S::NT >= 64) {
+    if constexpr (rows_direct<T, L, WIDE>()) {
         if (nrows == LPB && len == L && off_fits<Cx<T>>((long long)(LPB - 1) * dist + L)) {
             // full tile (uniform): unpredicated loads off one base
             const uint32_t off = (uint32_t)(b * dist + j) * (uint32_t)sizeof(Cx<T>);
@@ -93,13 +101,13 @@ __device__ __forceinline__ void tile_load(Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds,
     }
 }
 
-template <typename T, int L, int LPB>
+template <typename T, int L, int LPB, bool WIDE = false>
 __device__ __forceinline__ void tile_store(const Cx<T> (&reg)[Sched<L>::R], Cx<T> *lds, Cx<T> *dst, long long dist,
                                            int len, int nrows, int b, int j)
 {
     using S = Sched<L>;
     constexpr int NTH = S::NT * LPB;
-    if constexpr (S::NT >= 64) {
+    if constexpr (rows_direct<T, L, WIDE>()) {
         if (nrows == LPB && len == L && off_fits<Cx<T>>((long long)(LPB - 1) * dist + L)) {
             const uint32_t off = (uint32_t)(b * dist + j) * (uint32_t)sizeof(Cx<T>);
 #pragma unroll
@@ -369,7 +377,11 @@ k_rowT(RowTArgs a)
     const int nrows = (a.nrows - r0) < LPB ? (a.nrows - r0) : LPB;
     const C *src = (const C *)a.in + t * a.in_dist + (long long)r0 * L;
     C reg[S::R];
-    tile_load<T, L, LPB>(reg, lds, src, L, L, nrows, b, j);
+    // rows shorter than a wave are staged through LDS here (WIDE): production
+    // runs them on the prefetching grid (k_rowT_pf), and this pass is the one
+    // that grid is compared with bit for bit (built with direct loads, its
+    // FP64 256-point rows differed from the grid's in the last bits)
+    tile_load<T, L, LPB, true>(reg, lds, src, L, L, nrows, b, j);
     if constexpr (TWIN) {
         // the table read from global memory; the same arithmetic as k_rowT_pf
         const uint32_t rr = b < nrows ? (uint32_t)(r0 + b) : 0u;

@@ -584,6 +584,7 @@ struct RowR2CArgs {
     const void *w;     // e^{-2 pi i k / (2 Lc)}, k < Lc
     double scale;
     int dbl;           // adjoint of invrealdft: bins 0 < k < Lc times 2
+    int lpb;           // rows per workgroup (0: the default of rowT_lpb)
 };
 
 template <typename T, int L, int LPB>

@@ -231,6 +231,15 @@ template <typename T, int L> constexpr bool rowr2c_fits()
     return rowT_fits<T, L>() && rowT_lpb<T, L>() % 2 == 0;
 }
 
+template <typename T, int L, int LPB> hipError_t rowT_r2c_go(const RowR2CArgs &a, hipStream_t s)
+{
+    unsigned gy, gz;
+    grid_trans(a.ntrans, gy, gz);
+    dim3 g((unsigned)((a.nrows / 2 + LPB / 2 - 1) / (LPB / 2)), gy, gz), b(Sched<L>::NT * LPB);
+    hipLaunchKernelGGL((k_rowT_r2c<T, L, LPB>), g, b, 0, s, a);
+    return hipGetLastError();
+}
+
 template <typename T> hipError_t rowT_r2c_impl(int L, const RowR2CArgs &a, hipStream_t s)
 {
     switch (L) {
@@ -238,11 +247,10 @@ template <typename T> hipError_t rowT_r2c_impl(int L, const RowR2CArgs &a, hipSt
     case LL:                                                                                                  \
         if constexpr (rowr2c_fits<T, LL>()) {                                                                 \
             constexpr int LPB = rowT_lpb<T, LL>();                                                            \
-            unsigned gy, gz;                                                                                  \
-            grid_trans(a.ntrans, gy, gz);                                                                     \
-            dim3 g((unsigned)((a.nrows / 2 + LPB / 2 - 1) / (LPB / 2)), gy, gz), b(Sched<LL>::NT * LPB);      \
-            hipLaunchKernelGGL((k_rowT_r2c<T, LL, LPB>), g, b, 0, s, a);                                      \
-            return hipGetLastError();                                                                         \
+            if constexpr (LPB >= 4) {                                                                         \
+                if (a.lpb == LPB / 2) return rowT_r2c_go<T, LL, LPB / 2>(a, s);                               \
+            }                                                                                                 \
+            return rowT_r2c_go<T, LL, LPB>(a, s);                                                             \
         }                                                                                                     \
         break;
         NFFT_POW2_LIST(NFFT_CASE)

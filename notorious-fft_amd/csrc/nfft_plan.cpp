@@ -978,6 +978,7 @@ bool R2C::init(int N_, int prec_)
     single = false;
     // the row pass of a two-launch four-step absorbs the unpack (k_rowT_r2c)
     if (!inner.init(L, prec, 2, env_int("NFFT_R2C_FUSE", 1) ? 0 : 2)) return false;
+    r2c_lpb = env_int("NFFT_R2C_LPB", 0);
     fused = inner.algo == ALG_FOUR && !inner.pctr && (inner.n1 % 2) == 0 && env_int("NFFT_R2C_FUSE", 1) != 0;
     scratch_per_line = L + inner.scratch_per_line;
     return true;
@@ -1023,7 +1024,7 @@ hipError_t R2C::fwd(const void *in, void *out, long long nlines, long long in_di
             c.lpb = inner.col_lpb;
             e = launch_col(prec, inner.n1, -1, COL_TWIDDLE, c, s);
             if (e != hipSuccess) return e;
-            RowR2CArgs r{Z, dst, inner.n1, L, out_dist, cnt, inner.tw_b, rtw, scale, dbl};
+            RowR2CArgs r{Z, dst, inner.n1, L, out_dist, cnt, inner.tw_b, rtw, scale, dbl, r2c_lpb};
             e = launch_rowT_r2c(prec, inner.n2, r, s);
             if (e != hipSuccess) return e;
             continue;

@@ -19,6 +19,7 @@ struct R2C {
     const void *rtw = nullptr;  // e^{-2 pi i k/N}, k < N/2
     C2C inner;                  // multi-pass path
     bool fused = false;         // forward: unpack fused into the four-step row pass
+    int r2c_lpb = 0;            // rows per workgroup of the fused row pass (0: default)
     long long scratch_per_line = 0;
     bool init(int N, int prec);
     void release();
