@@ -652,26 +652,40 @@ __global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT_r2c(RowR2CArgs a)
     const long long Lc = (long long)n1 * L;
     C *dst = (C *)a.out + t * a.out_dist;
     const T sc = (T)a.scale, sk = a.dbl ? (T)2 * sc : sc;
-    for (int idx = threadIdx.x; idx < LPB * L; idx += NTH) {
-        const int slot = idx % LPB, k2 = idx / LPB;
-        const int r = row_of(slot);
-        if (r < 0) continue;
-        int ps = slot ^ 1, pc = L - 1 - k2;
-        if (r == 0) {
-            ps = slot;
-            pc = (L - k2) & (L - 1);
-        } else if (r == npairs) {
-            ps = slot;
-        }
-        const C zk = lds[slot * PADL + lpad<T>(k2)], zm = lds[ps * PADL + lpad<T>(pc)];
-        const long long k = (long long)k2 * n1 + r;
+    // Output k = k2 n1 + r.  Thread (slot, k0) keeps its row slot and walks
+    // k2 = k0 + KS i: row, partner slot and row factor are per-thread
+    // constants, and consecutive lanes store consecutive rows of one k2
+    // (runs of LPB / 2 points on each side of the pair).
+    constexpr int KS = NTH / LPB;
+    static_assert(NTH % LPB == 0 && L % KS == 0, "output sweep");
+    const int slot = threadIdx.x % LPB, k0 = threadIdx.x / LPB;
+    const int r = row_of(slot);
+    if (r < 0) return;  // past the last pair (no barrier follows)
+    // partner Z[Lc - k]: row n1 - r (slot ^ 1) at column L - 1 - k2; row 0
+    // pairs with itself at column -k2 mod L, row n1/2 at column L - 1 - k2
+    const int ps = (r == 0 || r == npairs) ? slot : (slot ^ 1);
+    const int pmask = r == 0 ? L - 1 : -1, pbase = r == 0 ? L : L - 1;
+    C wr = czero<T>();
+    if constexpr (WLDS) wr = wtab[L + slot];
+    const C *zrow = lds + slot * PADL, *prow = lds + ps * PADL;
+    const bool fast = off_fits<C>(Lc + 1);
+    const auto rd = buf_rsrc(dst);
+    const uint32_t voff = (uint32_t)((long long)k0 * n1 + r) * (uint32_t)sizeof(C);
+    const uint32_t ist = (uint32_t)((long long)KS * n1) * (uint32_t)sizeof(C);
+#pragma unroll 4
+    for (int i = 0; i < L / KS; i++) {
+        const int k2 = k0 + KS * i;
+        const C zk = zrow[lpad<T>(k2)], zm = prow[lpad<T>((pbase - k2) & pmask)];
         C wk;
-        if constexpr (WLDS) wk = cmul(wtab[k2], wtab[L + slot]);
-        else wk = w[k];
+        if constexpr (WLDS) wk = cmul(wtab[k2], wr);
+        else wk = w[(long long)k2 * n1 + r];
         C xk, xl;
         unpack_pair(zk, zm, wk, xk, xl);
-        dst[k] = cscale(xk, k == 0 ? sc : sk);
-        if (k == 0) dst[Lc] = cscale(xl, sc);
+        const bool k_is_0 = r == 0 && k2 == 0;
+        xk = cscale(xk, k_is_0 ? sc : sk);
+        if (fast) buf_st(rd, voff, (uint32_t)i * ist, xk);
+        else dst[(long long)k2 * n1 + r] = xk;
+        if (k_is_0) dst[Lc] = cscale(xl, sc);
     }
 }
 

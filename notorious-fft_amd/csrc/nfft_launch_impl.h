@@ -231,6 +231,14 @@ template <typename T, int L> constexpr bool rowr2c_fits()
     return rowT_fits<T, L>() && rowT_lpb<T, L>() % 2 == 0;
 }
 
+// twice the rows of rowT_lpb where the row pairs' tile still fits the LDS
+// and 1024 threads (longer output runs on both sides of each pair)
+template <typename T, int L, int LPB> constexpr bool rowr2c_wide_fits()
+{
+    return Sched<L>::NT * LPB <= 1024 && Sched<L>::S > 1 &&
+           (size_t)(LPB * lpadlen<T, L>() + L + LPB) * sizeof(Cx<T>) <= 150 * 1024;
+}
+
 template <typename T, int L, int LPB> hipError_t rowT_r2c_go(const RowR2CArgs &a, hipStream_t s)
 {
     unsigned gy, gz;
@@ -249,6 +257,9 @@ template <typename T> hipError_t rowT_r2c_impl(int L, const RowR2CArgs &a, hipSt
             constexpr int LPB = rowT_lpb<T, LL>();                                                            \
             if constexpr (LPB >= 4) {                                                                         \
                 if (a.lpb == LPB / 2) return rowT_r2c_go<T, LL, LPB / 2>(a, s);                               \
+            }                                                                                                 \
+            if constexpr (rowr2c_wide_fits<T, LL, 2 * LPB>()) {                                               \
+                if (a.lpb == 2 * LPB) return rowT_r2c_go<T, LL, 2 * LPB>(a, s);                               \
             }                                                                                                 \
             return rowT_r2c_go<T, LL, LPB>(a, s);                                                             \
         }                                                                                                     \
