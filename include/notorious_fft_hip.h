@@ -63,7 +63,12 @@ enum { NFFT_NORM_REFERENCE = 0, NFFT_NORM_NONE = 1, NFFT_NORM_INVERSE = 2 };
  *   REALDFT  real prod(dims) <-> complex prod(dims[0..d-2]) * (dims[d-1]/2+1)
  *   T2T3/T4  prod(dims) reals
  * Size rules follow the reference (src/07_legacy.h:1405-1634): DFT accepts
- * any N >= 1 per axis; REALDFT, T2T3 and T4 need powers of two. */
+ * any N >= 1 per axis; REALDFT, T2T3 and T4 need powers of two.
+ * Autotuning: records in the file named by NFFT_TUNE_CACHE choose, per
+ * precision and length, mixed radix or Bluestein for 7-smooth lengths and
+ * the row length and row pass of power-of-two four-steps; with NFFT_TUNE=1 a
+ * length without a record is measured at plan creation and its record
+ * appended. */
 int nfft_plan_create(nfft_plan **plan, int kind, int prec, int d, const int *dims,
                      long long batch, int norm);
 int nfft_plan_destroy(nfft_plan *plan);
@@ -133,9 +138,11 @@ int nfft_plan_check(nfft_plan *plan, void *stream);
  * (after a one-node counter reset), recorded into a HIP graph on first use
  * for each (in, out, direction) and replayed afterwards.  Inverse transforms
  * follow the reference's normalisation (1/N for non-power-of-two N only).
- * flags bit 0: autotune the persistent grid (workgroups per CU) on a scratch
- * batch; the choice is appended to the file named by NFFT_TUNE_CACHE, if set,
- * and read back by later creates with the same size list. */
+ * flags bit 0: autotune on a scratch batch -- first the tile size (column /
+ * row groups per tile) and the pipeline lag of the long transforms over five
+ * candidates, then the persistent grid (workgroups per CU); the choices are
+ * appended to the file named by NFFT_TUNE_CACHE, if set, and read back (not
+ * measured again) by later creates with the same size list. */
 int nfft_ragged_create(nfft_ragged **r, int prec, int count, const int *sizes, int flags);
 int nfft_ragged_execute(nfft_ragged *r, int inverse, const void *in, void *out, void *stream);
 int nfft_ragged_destroy(nfft_ragged *r);

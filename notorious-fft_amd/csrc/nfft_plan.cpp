@@ -370,6 +370,104 @@ static int tune_smooth(int n, int prec)
     return pick;
 }
 
+// Four-step choice: rows of 2^lg2 points, row pass on the prefetching grid
+// (pf = 1) or one workgroup per tile (pf = 0).  A record in NFFT_TUNE_CACHE
+// ("c2c-four p<prec> n<N> <lg2> <pf>") overrides the default rule; the tuner
+// below writes one.  The tuner's own candidates are built with t_four_force
+// set, so they neither read nor write records.
+struct FourChoice {
+    int lg2 = -1, pf = -1;
+};
+static thread_local FourChoice t_four_force;
+
+static std::string four_key(int prec, long long n)
+{
+    return "c2c-four p" + std::to_string(prec) + " n" + std::to_string(n);
+}
+
+static FourChoice tuned_four(int prec, long long n)
+{
+    FourChoice c;
+    const char *path = getenv("NFFT_TUNE_CACHE");
+    if (!path || !*path) return c;
+    FILE *f = fopen(path, "r");
+    if (!f) return c;
+    const std::string key = four_key(prec, n);
+    char line[256];
+    while (fgets(line, sizeof line, f)) {
+        int a = -1, b = -1;
+        if (strncmp(line, key.c_str(), key.size()) == 0 && line[key.size()] == ' ' &&
+            sscanf(line + key.size() + 1, "%d %d", &a, &b) == 2) {
+            c.lg2 = a;
+            c.pf = b;
+        }
+    }
+    fclose(f);
+    return c;
+}
+
+// Project-written autotuner for power-of-two four-steps: times every row
+// length the kernels have (2^8..2^11 points, columns within their limit) with
+// both row passes on a synthetic batch of about 2^24 (FP32) / 2^23 (FP64)
+// points and records the fastest in NFFT_TUNE_CACHE.  Returns the choice
+// (lg2 < 0: nothing measured).
+static FourChoice tune_four(int n, int prec)
+{
+    FourChoice best;
+    const long long lines = std::max(1LL, (prec == F32 ? (1LL << 24) : (1LL << 23)) / n);
+    const size_t cb = cx_bytes(prec);
+    hipStream_t st = setup_stream();
+    if (!st) return best;
+    void *x = nullptr, *y = nullptr, *scr = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = dev_malloc(&x, (size_t)lines * n * cb) == hipSuccess && dev_malloc(&y, (size_t)lines * n * cb) == hipSuccess &&
+              dev_malloc(&scr, (size_t)lines * n * cb) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+              hipEventCreate(&e1) == hipSuccess && launch_fill_uniform(x, 2 * lines * n, 5, 0, prec, st) == hipSuccess;
+    float best_ms = 1e30f;
+    const int lg = ilog2ll(n);
+    for (int lg2 = 8; lg2 <= 11 && ok; lg2++) {
+        const long long n2 = 1LL << lg2, n1 = n / n2;
+        if (n2 > rowT_max(prec) || n1 < 2 || n1 > col_max(prec) || lg2 >= lg) continue;
+        for (int pf = 1; pf >= 0 && ok; pf--) {
+            C2C c;
+            t_four_force = FourChoice{lg2, pf};
+            const bool built = c.init(n, prec, 0, 0);
+            t_four_force = FourChoice{};
+            if (!built || c.algo != ALG_FOUR) {
+                c.release();
+                continue;
+            }
+            float ms = 0;
+            for (int w = 0; w < 2 && ok; w++) ok = c.run(x, y, lines, n, n, -1, 1.0, scr, lines, st) == hipSuccess;
+            ok = ok && hipEventRecord(e0, st) == hipSuccess;
+            for (int r = 0; r < 5 && ok; r++) ok = c.run(x, y, lines, n, n, -1, 1.0, scr, lines, st) == hipSuccess;
+            ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+                 hipEventElapsedTime(&ms, e0, e1) == hipSuccess;
+            c.release();
+            if (ok && ms < best_ms) {
+                best_ms = ms;
+                best = FourChoice{lg2, pf};
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (void *p : {x, y, scr})
+        if (p) (void)hipFree(p);
+    if (!ok) {
+        (void)hipGetLastError();
+        return FourChoice{};
+    }
+    const char *path = getenv("NFFT_TUNE_CACHE");
+    if (best.lg2 >= 0 && path && *path) {
+        if (FILE *f = fopen(path, "a")) {
+            fprintf(f, "%s %d %d\n", four_key(prec, n).c_str(), best.lg2, best.pf);
+            fclose(f);
+        }
+    }
+    return best;
+}
+
 // Dependency waits of the persistent four-step give up after this long
 // (wall clock); a healthy wait lasts microseconds.
 static const double kPersistTimeoutSec = 2.0;
@@ -503,10 +601,19 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
             const int lg = ilog2ll(n);
             int lg2 = (lg - 10 >= 4 && lg - 10 <= 11) ? 10 : lg / 2;
             lg2 = std::min(lg2, ilog2ll(rowT_max(prec)));
+            // a tuned choice (NFFT_TUNE_CACHE record, or measured now with
+            // NFFT_TUNE=1), then NFFT_FOUR_N2, over the rule above
+            FourChoice fc = t_four_force;
+            if (fc.lg2 < 0) {
+                fc = tuned_four(prec, n);
+                if (fc.lg2 < 0 && env_int("NFFT_TUNE", 0)) fc = tune_four(n, prec);
+            }
+            const auto split_ok = [&](long long f) {
+                return f > 0 && is_pow2(f) && f <= rowT_max(prec) && n / f <= col_max(prec) && n / f >= 2;
+            };
+            if (fc.lg2 >= 0 && fc.lg2 < 31 && split_ok(1LL << fc.lg2)) lg2 = fc.lg2;
             const int forced = env_int("NFFT_FOUR_N2", 0);
-            if (forced > 0 && is_pow2(forced) && forced <= rowT_max(prec) && n / forced <= col_max(prec) &&
-                n / forced >= 2)
-                lg2 = ilog2ll(forced);
+            if (split_ok(forced)) lg2 = ilog2ll(forced);
             n2 = 1 << lg2;
             n1 = n / n2;
             if (n1 > col_max(prec)) {
@@ -515,7 +622,7 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
             }
             col_lpb = env_int("NFFT_COL_LPB", 0);
             row_lpb = env_int("NFFT_ROWT_LPB", 0);
-            row_pf = env_int("NFFT_ROWT_PF", 1) != 0 ? 1 : 0;
+            row_pf = env_int("NFFT_ROWT_PF", fc.pf >= 0 ? fc.pf : 1) != 0 ? 1 : 0;
             // FP32 1024-point rows take the inter-pass twiddle (column pass
             // 992 -> 910 us per 2^28 points; the prefetching row grid absorbs it)
             row_twin = prec == F32 && n2 == 1024 && env_int("NFFT_ROW_TWIDDLE", 1) != 0 ? 1 : 0;

@@ -500,6 +500,13 @@ int env_int_rg(const char *name, int dflt)
     const char *v = getenv(name);
     return v && *v ? atoi(v) : dflt;
 }
+// Tile size (column/row groups per tile) and pipeline-lag scale (quarters of
+// the default lag) of the batch being created: set by the autotuner around
+// its candidate builds and from a tuning record (0: the defaults).
+struct RagShape {
+    int gpt = 0, lag4 = 0;
+};
+thread_local RagShape t_rag_shape;
 // splits of the long transforms (see the kernel's dispatch table)
 bool four_split(int n, int *n1, int *n2)
 {
@@ -617,9 +624,7 @@ static void tune_store(const std::string &key, int val)
 // allocation failed since `fails0` (alloc_failures()), else NFFT_E_HIP.
 static int setup_fail(long long fails0) { return alloc_failures() > fails0 ? NFFT_E_ALLOC : NFFT_E_HIP; }
 
-extern "C" {
-
-int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes, int flags)
+static int ragged_create_impl(nfft_ragged **out, int prec, int count, const int *sizes, int flags)
 {
     clear_error();
     const long long fails0 = alloc_failures();
@@ -804,7 +809,7 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
     std::stable_sort(longs.begin(), longs.end(), [](const Long &a, const Long &b) { return a.cost > b.cost; });
     // column/row groups per tile: 16 measured best on the 10,000-size batch
     // (FP32 4 / 8 / 16 / 32 groups: 2.65 / 2.55 / 2.52 / 2.61 ms)
-    const int gpt = std::max(1, env_int_rg("NFFT_RAGGED_GPT", 16));
+    const int gpt = std::max(1, env_int_rg("NFFT_RAGGED_GPT", t_rag_shape.gpt > 0 ? t_rag_shape.gpt : 16));
     long long slot_elems = 0;
     for (Long &l : longs) {
         for (int ph = 0; ph < l.phases; ph++) l.tiles[ph] = (l.groups[ph] + gpt - 1) / gpt;
@@ -827,7 +832,8 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
         int dev0 = 0, ncu0 = 256;
         (void)hipGetDevice(&dev0);
         (void)hipDeviceGetAttribute(&ncu0, hipDeviceAttributeMultiprocessorCount, dev0);
-        const int dflt = std::max(16, ncu0 * (prec == F32 ? 4 : 2) * 3 / 4);
+        int dflt = std::max(16, ncu0 * (prec == F32 ? 4 : 2) * 3 / 4);
+        if (t_rag_shape.lag4 > 0) dflt = std::max(16, dflt * t_rag_shape.lag4 / 4);
         const long long slot_bytes = slot_elems * (long long)cb;
         const long long want_mb = std::max(512LL, std::min(4096LL, (4LL * dflt * slot_bytes) >> 20));
         const long long budget = (long long)std::max(16, env_int_rg("NFFT_RAGGED_MB", (int)want_mb)) << 20;
@@ -930,6 +936,7 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
     }
     const std::string key = tune_key(prec, std::vector<int>(sizes, sizes + count));
     int per_cu = tune_lookup(key);
+    const bool recorded = per_cu > 0;
     if (per_cu <= 0) per_cu = prec == F32 ? 4 : 2;
     r->grid = std::max(1, std::min(r->nitems, ncu * per_cu));
     r->slot_elems = slot_elems;
@@ -951,7 +958,7 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
         ragged_free(r);
         return NFFT_E_ALLOC;
     }
-    if (flags & 1) {
+    if ((flags & 1) && !recorded) {
         // project-written autotuner: time the persistent launch for a few
         // workgroups-per-CU choices on a scratch batch and keep the fastest
         void *tin = nullptr, *tout = nullptr;
@@ -987,6 +994,92 @@ int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes,
     }
     *out = r;
     return NFFT_OK;
+}
+
+// Project-written autotuner for the tile size and the pipeline lag: builds
+// the batch with each candidate (16 groups per tile and the default lag, and
+// the neighbours measured on the 10,000-size batch), times four launches of
+// each on a scratch batch and keeps the fastest ("ragged-gl" record).
+static int ragged_tune_shape(int prec, int count, const int *sizes, const std::string &key, RagShape *best)
+{
+    const size_t cb = cx_bytes(prec);
+    long long total = 0;
+    for (int i = 0; i < count; i++) total += sizes[i];
+    void *tin = nullptr, *tout = nullptr;
+    if (dev_malloc(&tin, (size_t)total * cb) != hipSuccess || dev_malloc(&tout, (size_t)total * cb) != hipSuccess) {
+        if (tin) (void)hipFree(tin);
+        return NFFT_E_ALLOC;
+    }
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    bool ok = hipStreamCreateWithFlags(&st, hipStreamNonBlocking) == hipSuccess && hipEventCreate(&e0) == hipSuccess &&
+              hipEventCreate(&e1) == hipSuccess && launch_fill_uniform(tin, 2 * total, 7, 0, prec, st) == hipSuccess;
+    float best_ms = 1e30f;
+    int fail = NFFT_E_HIP;
+    const RagShape cands[] = {{16, 4}, {8, 4}, {32, 4}, {16, 2}, {16, 8}};
+    for (const RagShape &c : cands) {
+        if (!ok) break;
+        t_rag_shape = c;
+        nfft_ragged *r = nullptr;
+        const int rc = ragged_create_impl(&r, prec, count, sizes, 0);
+        t_rag_shape = RagShape{};
+        if (rc != NFFT_OK) {
+            if (rc == NFFT_E_ALLOC) continue;  // this shape's slots did not fit
+            fail = rc;
+            ok = false;
+            break;
+        }
+        float ms = 0;
+        unsigned ew[2] = {1, 1};
+        ok = launch_ragged(r, 0, tin, tout, st) == NFFT_OK && hipEventRecord(e0, st) == hipSuccess;
+        for (int k = 0; k < 4 && ok; k++) ok = launch_ragged(r, 0, tin, tout, st) == NFFT_OK;
+        ok = ok && hipEventRecord(e1, st) == hipSuccess && hipEventSynchronize(e1) == hipSuccess &&
+             hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+             hipMemcpy(ew, r->ctr + 1, sizeof ew, hipMemcpyDeviceToHost) == hipSuccess && ew[0] == 0 && ew[1] == 0;
+        ragged_free(r);
+        if (ok && ms < best_ms) {
+            best_ms = ms;
+            *best = c;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    (void)hipFree(tin);
+    (void)hipFree(tout);
+    if (!ok || best_ms >= 1e30f) {
+        (void)hipGetLastError();
+        return fail;
+    }
+    tune_store(key, best->gpt * 100 + best->lag4);
+    return NFFT_OK;
+}
+
+extern "C" {
+
+int nfft_ragged_create(nfft_ragged **out, int prec, int count, const int *sizes, int flags)
+{
+    clear_error();
+    if (!out || count < 1 || !sizes || (prec != NFFT_F32 && prec != NFFT_F64))
+        return ragged_create_impl(out, prec, count, sizes, flags);  // reports the argument error
+    // tile size and lag: a tuning record, else measured with flags bit 0
+    RagShape shape;
+    const std::string key = "ragged-gl" + tune_key(prec, std::vector<int>(sizes, sizes + count)).substr(6);
+    const int rec = tune_lookup(key);
+    if (rec > 0) {
+        shape.gpt = rec / 100;
+        shape.lag4 = rec % 100;
+    } else if (flags & 1) {
+        // the sizes are validated by the first candidate build
+        const int rc = ragged_tune_shape(prec, count, sizes, key, &shape);
+        if (rc == NFFT_E_SIZE || rc == NFFT_E_UNSUPPORTED || rc == NFFT_E_INVALID) return rc;
+        if (rc != NFFT_OK) shape = RagShape{};  // nothing measured: the defaults
+        clear_error();
+    }
+    t_rag_shape = shape;
+    const int rc = ragged_create_impl(out, prec, count, sizes, flags);
+    t_rag_shape = RagShape{};
+    return rc;
 }
 
 int nfft_ragged_execute(nfft_ragged *r, int inverse, const void *in, void *out, void *stream)

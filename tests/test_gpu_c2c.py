@@ -316,6 +316,39 @@ def test_mixed_radix_four_step(pkg, dev, prec, n, monkeypatch):
     p.close()
 
 
+@pytest.mark.parametrize("prec,lg", [("f32", 16), ("f64", 18)])
+def test_four_step_autotuner(pkg, dev, tmp_path, monkeypatch, prec, lg):
+    """NFFT_TUNE=1 times every row length (256..2048 points) with both row
+    passes at plan creation and records the fastest as
+    "c2c-four p<prec> n<N> <log2 rows> <prefetching grid>"; later plans
+    follow the record (also one written by hand), and every choice matches
+    the oracle."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("NFFT_TUNE_CACHE", str(cache))
+    monkeypatch.setenv("NFFT_TUNE", "1")
+    n, batch = 1 << lg, 4
+    x = rounded(ol.uniform(17, 2 * n * batch), prec)
+    want = ol.run_oracle("dft", [n], x[:2 * n])
+    got = run_plan(pkg, dev, "dft", [n], x, batch, prec)
+    assert ol.rel_l2(got[:2 * n], want) < tol_for(prec, n)
+    lines = [l.split() for l in cache.read_text().splitlines()]
+    pk = "p0" if prec == "f32" else "p1"
+    rec = [l for l in lines if l[:3] == ["c2c-four", pk, f"n{n}"]]
+    assert len(rec) == 1 and 8 <= int(rec[0][3]) <= 11 and rec[0][4] in ("0", "1"), lines
+    monkeypatch.setenv("NFFT_TUNE", "0")
+    p = pkg.Plan("dft", prec, [n], batch)
+    assert f"x{1 << int(rec[0][3])}" in p.describe(), (p.describe(), rec)
+    p.close()
+    assert len(cache.read_text().splitlines()) == len(lines)  # no second measurement
+    # a record written by hand: 256-point rows on the one-workgroup-per-tile pass
+    cache.write_text(f"c2c-four {pk} n{n} 8 0\n")
+    p = pkg.Plan("dft", prec, [n], batch)
+    assert p.describe().startswith(f"four-step {n // 256}x256"), p.describe()
+    p.close()
+    got = run_plan(pkg, dev, "dft", [n], x, batch, prec)
+    assert ol.rel_l2(got[:2 * n], want) < tol_for(prec, n)
+
+
 def test_smooth_autotuner(pkg, dev, tmp_path, monkeypatch):
     """NFFT_TUNE=1 times mixed radix against Bluestein at plan creation and
     records the winner in NFFT_TUNE_CACHE; later plans follow the record,

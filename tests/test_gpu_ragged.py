@@ -41,10 +41,33 @@ def test_ragged_slot_reuse(pkg, dev, prec, mb, gpt, lag, monkeypatch):
     check_vs_pocketfft(pkg, dev, prec, sizes)
 
 
-def check_vs_pocketfft(pkg, dev, prec, sizes):
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_ragged_autotuner(pkg, dev, prec, tmp_path, monkeypatch):
+    """flags bit 0: the tile size and pipeline lag are measured over five
+    candidates (one "ragged-gl" record, value groups x 100 + lag quarters),
+    then the workgroups per CU ("ragged" record); a second create with the
+    same sizes reads both records and measures nothing; both batches match
+    pocketfft."""
+    cache = tmp_path / "tune.txt"
+    monkeypatch.setenv("NFFT_TUNE_CACHE", str(cache))
+    sizes = ragged_sizes(120, seed=4) + [65521, 65536, 40000, 30030]
+    check_vs_pocketfft(pkg, dev, prec, sizes, flags=1)
+    lines = [l.split() for l in cache.read_text().splitlines()]
+    pk = "p0" if prec == "f32" else "p1"
+    gl = [l for l in lines if l[0] == "ragged-gl" and l[1] == pk]
+    pc = [l for l in lines if l[0] == "ragged" and l[1] == pk]
+    assert len(gl) == 1 and len(pc) == 1, lines
+    v = int(gl[0][4])
+    assert (v // 100, v % 100) in ((16, 4), (8, 4), (32, 4), (16, 2), (16, 8)), gl
+    assert 1 <= int(pc[0][4]) <= 4, pc
+    check_vs_pocketfft(pkg, dev, prec, sizes, flags=1)
+    assert len(cache.read_text().splitlines()) == len(lines)  # read back, not measured again
+
+
+def check_vs_pocketfft(pkg, dev, prec, sizes, flags=0):
     import torch
     lib = pkg.core()
-    h = make(pkg, prec, sizes)
+    h = make(pkg, prec, sizes, flags)
     total = lib.nfft_ragged_total(h)
     assert total == sum(sizes)
     x = rounded(ol.uniform(51, 2 * total), prec)
