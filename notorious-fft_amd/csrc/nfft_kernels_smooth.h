@@ -155,7 +155,7 @@ __device__ __forceinline__ void smooth_stage_any(const Cx<T> *src, Cx<T> *dst, i
 // of the ping-pong); returns the buffer holding the result.  All threads of
 // the workgroup must call it.
 template <typename T, int DIR>
-__device__ Cx<T> *smooth_lines(Cx<T> *a, Cx<T> *b, int G, const SmoothPlan &sp, const Cx<T> *tw)
+__device__ __forceinline__ Cx<T> *smooth_lines(Cx<T> *a, Cx<T> *b, int G, const SmoothPlan &sp, const Cx<T> *tw)
 {
     Cx<T> *src = a, *dst = b;
     int Ns = 1;
@@ -245,7 +245,7 @@ namespace nfft {
 // other workgroups consume within the same launch).
 // ---------------------------------------------------------------------------
 template <typename T, int DIR, bool COH>
-__device__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, const SmoothPlan &sp1, int N2, int c0, int w,
+__device__ __forceinline__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, const SmoothPlan &sp1, int N2, int c0, int w,
                                 const Cx<T> *tw1, const Tw2 &tw2, Cx<T> *A, Cx<T> *B)
 {
     using C = Cx<T>;
@@ -269,7 +269,7 @@ __device__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, const SmoothPlan &s
 }
 
 template <typename T, int DIR, bool COH>
-__device__ void smooth_row_tile(const Cx<T> *scr, Cx<T> *out, const SmoothPlan &sp2, int N1, int r0, int w,
+__device__ __forceinline__ void smooth_row_tile(const Cx<T> *scr, Cx<T> *out, const SmoothPlan &sp2, int N1, int r0, int w,
                                 const Cx<T> *tw2t, T scale, Cx<T> *A, Cx<T> *B)
 {
     using C = Cx<T>;

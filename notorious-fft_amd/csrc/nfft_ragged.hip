@@ -92,8 +92,12 @@ template <int L> constexpr int rg_lines() { return RG_THREADS / Sched<L>::NT; }
 // groups of short lines
 // ---------------------------------------------------------------------------
 template <typename T, int L, int DIR, bool BLUE>
-__device__ __noinline__ void rg_line(const RagArgs &a, const RagItem &it, Cx<T> *lds, long long *bases)
+__device__ __noinline__ void rg_line(const RagArgs &a, const RagItem &it, Cx<T> *, long long *bases)
 {
+    // the LDS through the kernel's own symbol (address space known: ds
+    // instructions rather than flat ones)
+    extern __shared__ char smem[];
+    Cx<T> *lds = (Cx<T> *)smem;
     using S = Sched<L>;
     using C = Cx<T>;
     constexpr int LPB = rg_lines<L>();
@@ -190,7 +194,7 @@ template <bool COH, typename C> __device__ __forceinline__ void rg_st(C *p, C v)
 // MODE 1: x[n] * chirp (n < nv, conj if cj) -> FFT -> * tw2 -> dst (scratch; Bluestein in)
 // MODE 2: src (scratch) -> IFFT -> y[n] = conj?(v * chirp[n]) * sc, n < nv (Bluestein out)
 template <typename T, int L1, int L2, int DIR, int MODE>
-__device__ void rg_cols(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, int nv, int cj, T sc, Cx<T> *lds, int g0,
+__device__ __forceinline__ void rg_cols(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, int nv, int cj, T sc, Cx<T> *lds, int g0,
                         int g1)
 {
     using S = Sched<L1>;
@@ -242,7 +246,7 @@ __device__ void rg_cols(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, int nv, 
 // MODE 0: scratch -> FFT_L2 (DIR) -> transposed store dst[r + L1*k] (four-step pass 2)
 // MODE 1: scratch -> FFT, * H[r][k], IFFT, * conj(tw2(r*n2)) -> scratch in place (Bluestein middle)
 template <typename T, int L1, int L2, int DIR, int MODE>
-__device__ void rg_rows(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, Cx<T> *lds, int g0, int g1)
+__device__ __forceinline__ void rg_rows(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, Cx<T> *lds, int g0, int g1)
 {
     using S = Sched<L2>;
     using C = Cx<T>;
@@ -303,8 +307,12 @@ __device__ void rg_rows(const Cx<T> *src, Cx<T> *dst, const RagTab &tb, Cx<T> *l
 
 // one tile of a long power-of-two transform (four-step N1 x N2)
 template <typename T, int N1, int N2, int DIR>
-__device__ __noinline__ void rg_four(const RagArgs &a, const RagItem &it, Cx<T> *lds)
+__device__ __noinline__ void rg_four(const RagArgs &a, const RagItem &it, Cx<T> *)
 {
+    // the LDS through the kernel's own symbol (address space known: ds
+    // instructions rather than flat ones)
+    extern __shared__ char smem[];
+    Cx<T> *lds = (Cx<T> *)smem;
     using C = Cx<T>;
     const RagTab tb = a.tabs[it.tab];
     const long long base = a.offs[a.ids[it.first]];
@@ -315,8 +323,12 @@ __device__ __noinline__ void rg_four(const RagArgs &a, const RagItem &it, Cx<T> 
 
 // one tile of a long Bluestein transform (three passes over M1 x M2)
 template <typename T, int M1, int M2, int DIR>
-__device__ __noinline__ void rg_blue3(const RagArgs &a, const RagItem &it, Cx<T> *lds)
+__device__ __noinline__ void rg_blue3(const RagArgs &a, const RagItem &it, Cx<T> *)
 {
+    // the LDS through the kernel's own symbol (address space known: ds
+    // instructions rather than flat ones)
+    extern __shared__ char smem[];
+    Cx<T> *lds = (Cx<T> *)smem;
     using C = Cx<T>;
     const RagTab tb = a.tabs[it.tab];
     const long long base = a.offs[a.ids[it.first]];
@@ -333,8 +345,12 @@ __device__ __noinline__ void rg_blue3(const RagArgs &a, const RagItem &it, Cx<T>
 
 // a group of short mixed-radix transforms (same length), LDS-resident
 template <typename T, int DIR>
-__device__ __noinline__ void rg_smooth_group(const RagArgs &a, const RagItem &it, Cx<T> *lds, long long *bases)
+__device__ __noinline__ void rg_smooth_group(const RagArgs &a, const RagItem &it, Cx<T> *, long long *bases)
 {
+    // the LDS through the kernel's own symbol (address space known: ds
+    // instructions rather than flat ones)
+    extern __shared__ char smem[];
+    Cx<T> *lds = (Cx<T> *)smem;
     using C = Cx<T>;
     const RagTab &tb = a.tabs[it.tab];
     const int L = tb.sp1.L, G = it.count;
@@ -358,8 +374,12 @@ __device__ __noinline__ void rg_smooth_group(const RagArgs &a, const RagItem &it
 
 // one tile of a long mixed-radix transform (four-step N1 x N2)
 template <typename T, int DIR>
-__device__ __noinline__ void rg_smooth4(const RagArgs &a, const RagItem &it, Cx<T> *lds)
+__device__ __noinline__ void rg_smooth4(const RagArgs &a, const RagItem &it, Cx<T> *)
 {
+    // the LDS through the kernel's own symbol (address space known: ds
+    // instructions rather than flat ones)
+    extern __shared__ char smem[];
+    Cx<T> *lds = (Cx<T> *)smem;
     using C = Cx<T>;
     const RagTab &tb = a.tabs[it.tab];
     const long long base = a.offs[a.ids[it.first]];
