@@ -628,7 +628,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *LPB) k_rowT_r2c(RowR2CArgs a)
         }
     }
     C reg[S::R];
-    if constexpr (S::NT >= 64) {
+    if constexpr (rows_direct<T, L>()) {
         const int r = row_of(b);
 #pragma unroll
         for (int q = 0; q < S::R; q++) reg[q] = r >= 0 ? src[(long long)r * L + j + S::NT * q] : czero<T>();

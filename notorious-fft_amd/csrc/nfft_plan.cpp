@@ -303,20 +303,46 @@ static std::string smooth_key(int prec, long long n)
     return "c2c-smooth p" + std::to_string(prec) + " n" + std::to_string(n);
 }
 
-// Choice recorded in the NFFT_TUNE_CACHE file (-1: none).
+// Tuning results of this process, by record key (value: the text after the
+// key), so that a length is measured once per process even when no
+// NFFT_TUNE_CACHE file is named.
+static std::mutex g_tune_mu;
+static std::map<std::string, std::string> g_tune_mem;
+
+static bool tune_mem_get(const std::string &key, std::string *val)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_mem.find(key);
+    if (it == g_tune_mem.end()) return false;
+    *val = it->second;
+    return true;
+}
+
+static void tune_mem_put(const std::string &key, const std::string &val)
+{
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_mem[key] = val;
+}
+
+// Choice recorded in the NFFT_TUNE_CACHE file (the last record wins), else
+// measured earlier in this process (-1: none).
 static int tuned_choice(int prec, long long n)
 {
-    const char *path = getenv("NFFT_TUNE_CACHE");
-    if (!path || !*path) return -1;
-    FILE *f = fopen(path, "r");
-    if (!f) return -1;
     const std::string key = smooth_key(prec, n);
-    char line[256];
     int found = -1;
-    while (fgets(line, sizeof line, f)) {
-        if (strncmp(line, key.c_str(), key.size()) == 0 && line[key.size()] == ' ') found = atoi(line + key.size() + 1);
+    const char *path = getenv("NFFT_TUNE_CACHE");
+    if (path && *path) {
+        if (FILE *f = fopen(path, "r")) {
+            char line[256];
+            while (fgets(line, sizeof line, f)) {
+                if (strncmp(line, key.c_str(), key.size()) == 0 && line[key.size()] == ' ')
+                    found = atoi(line + key.size() + 1);
+            }
+            fclose(f);
+        }
     }
-    fclose(f);
+    std::string mem;
+    if (found < 0 && tune_mem_get(key, &mem)) found = atoi(mem.c_str());
     return found;
 }
 
@@ -360,6 +386,7 @@ static int tune_smooth(int n, int prec)
         return -1;
     }
     const int pick = ms[1] < ms[0] ? 1 : 0;
+    tune_mem_put(smooth_key(prec, n), std::to_string(pick));
     const char *path = getenv("NFFT_TUNE_CACHE");
     if (path && *path) {
         if (FILE *f = fopen(path, "a")) {
@@ -388,21 +415,24 @@ static std::string four_key(int prec, long long n)
 static FourChoice tuned_four(int prec, long long n)
 {
     FourChoice c;
-    const char *path = getenv("NFFT_TUNE_CACHE");
-    if (!path || !*path) return c;
-    FILE *f = fopen(path, "r");
-    if (!f) return c;
     const std::string key = four_key(prec, n);
-    char line[256];
-    while (fgets(line, sizeof line, f)) {
-        int a = -1, b = -1;
-        if (strncmp(line, key.c_str(), key.size()) == 0 && line[key.size()] == ' ' &&
-            sscanf(line + key.size() + 1, "%d %d", &a, &b) == 2) {
-            c.lg2 = a;
-            c.pf = b;
+    const char *path = getenv("NFFT_TUNE_CACHE");
+    if (path && *path) {
+        if (FILE *f = fopen(path, "r")) {
+            char line[256];
+            while (fgets(line, sizeof line, f)) {
+                int a = -1, b = -1;
+                if (strncmp(line, key.c_str(), key.size()) == 0 && line[key.size()] == ' ' &&
+                    sscanf(line + key.size() + 1, "%d %d", &a, &b) == 2) {
+                    c.lg2 = a;
+                    c.pf = b;
+                }
+            }
+            fclose(f);
         }
     }
-    fclose(f);
+    std::string mem;
+    if (c.lg2 < 0 && tune_mem_get(key, &mem) && sscanf(mem.c_str(), "%d %d", &c.lg2, &c.pf) != 2) c = FourChoice{};
     return c;
 }
 
@@ -469,6 +499,7 @@ static FourChoice tune_four(int n, int prec, FourChoice dflt)
         return FourChoice{};
     }
     if (dflt_ms <= 1.02f * best_ms) best = dflt;
+    if (best.lg2 >= 0) tune_mem_put(four_key(prec, n), std::to_string(best.lg2) + " " + std::to_string(best.pf));
     const char *path = getenv("NFFT_TUNE_CACHE");
     if (best.lg2 >= 0 && path && *path) {
         if (FILE *f = fopen(path, "a")) {
@@ -1094,9 +1125,24 @@ bool R2C::init(int N_, int prec_)
         return tw != nullptr;
     }
     single = false;
-    // the row pass of a two-launch four-step absorbs the unpack (k_rowT_r2c)
-    if (!inner.init(L, prec, 2, env_int("NFFT_R2C_FUSE", 1) ? 0 : 2)) return false;
-    r2c_lpb = env_int("NFFT_R2C_LPB", 0);
+    // the row pass of a two-launch four-step absorbs the unpack (k_rowT_r2c).
+    // Its row length: shorter rows in row-pair tiles of twice the height
+    // (128-byte output runs on both sides of each pair) where measured faster
+    // on 2^28 reals -- FP64 256-point rows up to N = 2^20 (2^16: 1.71 ->
+    // 1.53 ms, 2^20: 1.80 -> 1.71 ms), FP32 512-point rows while the columns
+    // stay within 512 points (2^16: 0.97 -> 0.90 ms, 2^19: 0.95 -> 0.91 ms) --
+    // and the four-step's own split otherwise.
+    const bool fuse = env_int("NFFT_R2C_FUSE", 1) != 0;
+    int lg2r = 0;
+    if (fuse && env_int("NFFT_FOUR_N2", 0) == 0) {
+        if (prec == F64 && L >= (2LL << 8) && L <= (1LL << 19)) lg2r = 8;
+        if (prec == F32 && L >= (2LL << 9) && L <= (512LL << 9)) lg2r = 9;
+    }
+    t_four_force = lg2r ? FourChoice{lg2r, 1} : FourChoice{};
+    const bool built = inner.init(L, prec, 2, fuse ? 0 : 2);
+    t_four_force = FourChoice{};
+    if (!built) return false;
+    r2c_lpb = env_int("NFFT_R2C_LPB", lg2r && inner.n2 == (1 << lg2r) ? 2 * rowT_lpb_rt(prec, 1 << lg2r) : 0);
     fused = inner.algo == ALG_FOUR && !inner.pctr && (inner.n1 % 2) == 0 && env_int("NFFT_R2C_FUSE", 1) != 0;
     scratch_per_line = L + inner.scratch_per_line;
     return true;
