@@ -1,5 +1,5 @@
 // nfft_k_xcd.hip -- instantiations and launcher of the XCD-affine FP32
-// four-step (k_four_xcd): N = N1 x 1024 with N1 in {512, 1024, 2048}, and the
+// four-step (k_four_xcd): N = N1 x 1024 with N1 in {64, ..., 2048}, and the
 // one-time probe of the XCC ids a device runs workgroups on.
 #include <algorithm>
 #include <mutex>
@@ -98,6 +98,9 @@ bool xcd_supported(long long n, int *n1, int *per_cu)
 {
     int occ = 0;
     switch (n) {
+    case 1LL << 16: *n1 = 64; occ = xcd_occ<64>(); break;
+    case 1LL << 17: *n1 = 128; occ = xcd_occ<128>(); break;
+    case 1LL << 18: *n1 = 256; occ = xcd_occ<256>(); break;
     case 1LL << 19: *n1 = 512; occ = xcd_occ<512>(); break;
     case 1LL << 20: *n1 = 1024; occ = xcd_occ<1024>(); break;
     case 1LL << 21: *n1 = 2048; occ = xcd_occ<2048>(); break;
@@ -112,6 +115,12 @@ hipError_t launch_four_xcd(int n1, int dir, bool twrow, const XcdArgs &a, int gr
     const int slot = prof_on() ? prof_begin(s) : -1;
     hipError_t e = hipErrorInvalidValue;
     switch (n1 * 2 + (twrow ? 1 : 0)) {
+    case 128: e = xcd_go<64, false>(dir, a, grid, s); break;
+    case 129: e = xcd_go<64, true>(dir, a, grid, s); break;
+    case 256: e = xcd_go<128, false>(dir, a, grid, s); break;
+    case 257: e = xcd_go<128, true>(dir, a, grid, s); break;
+    case 512: e = xcd_go<256, false>(dir, a, grid, s); break;
+    case 513: e = xcd_go<256, true>(dir, a, grid, s); break;
     case 1024: e = xcd_go<512, false>(dir, a, grid, s); break;
     case 1025: e = xcd_go<512, true>(dir, a, grid, s); break;
     case 2048: e = xcd_go<1024, false>(dir, a, grid, s); break;

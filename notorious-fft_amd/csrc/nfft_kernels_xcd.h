@@ -17,9 +17,15 @@
 // hardware register XCC_ID and takes its work from that XCD's queue, so the
 // producer and the consumer of an intermediate always share an L2, whatever
 // the dispatcher did.  Within one XCD the L2 is the coherence point: the
-// intermediate is written with plain (write-back) stores and read with plain
-// loads after the reading CU has invalidated its L1 (buffer_inv sc0); no L2
-// write-back or invalidate is needed.  The host maps the XCC ids the device
+// intermediate is written with plain (write-back) stores and read with
+// agent-scope loads (cache-policy bit sc1: served by the L2, never by the
+// reading CU's L1, as the compiler emits for an agent-scope atomic load); no
+// L2 write-back is needed.  Plain loads after a workgroup-scope invalidate
+// (buffer_inv sc0) are not enough: that does not drop the CU's L1 lines, so
+// a row tile could read lines that an earlier user of the same slot left in
+// that L1 (whole rows of stale points, seen when a slot is reused within the
+// L1's lifetime: N1 <= 128).  An agent-scope invalidate (buffer_inv sc1) is
+// correct but costs 25 % at N = 2^20 (1.63 -> 2.03 ms).  The host maps the XCC ids the device
 // reports to queues 0..nq-1 (xcd_topology); a workgroup on an unmapped id
 // raises the error flag and does nothing.
 //
@@ -61,6 +67,7 @@
 namespace nfft {
 
 constexpr int kXcdQueues = 8;   // queues (XCDs) at most
+constexpr int kCpAgent = 16;    // cache-policy bit sc1: agent-scope (L1-bypassing) access
 constexpr int kXcdSlots = 2;    // intermediate slots per queue
 constexpr int kXcdIds = 16;     // XCC_ID values the map covers
 
@@ -253,14 +260,12 @@ __global__ void __launch_bounds__(XShape<N1>::BT, 1) k_four_xcd(XcdArgs a)
         } else if (kind == XI_ROW) {
             const int rb = tid / S2::NT, rj = tid % S2::NT;
             const uint32_t off = (uint32_t)(rb * N2 + rj) * (uint32_t)sizeof(C);
-            // no line of the slot from an earlier use may be served by this CU's L1
-            asm volatile("buffer_inv sc0" ::: "memory");
             const C *slot = (const C *)a.scratch + ((long long)q * kXcdSlots + (s % kXcdSlots)) * N;
             const auto rs = buf_rsrc(slot + (long long)tile * LPB2 * N2);
 #pragma unroll
             for (int l = 0; l < NL; l++)
 #pragma unroll
-                for (int i = 0; i < R; i++) v[l][i] = buf_ld<C, CPI>(rs, off, l * rlst + i * rqst);
+                for (int i = 0; i < R; i++) v[l][i] = buf_ld<C, CPI | kCpAgent>(rs, off, l * rlst + i * rqst);
         }
     };
 

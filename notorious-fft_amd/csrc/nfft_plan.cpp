@@ -556,10 +556,12 @@ static bool persist_setup(C2C &c)
 static const long long kXcdCap = 1LL << 12;
 
 // The XCD-affine four-step for FP32 lengths that have an instance, where it
-// is the default: N = 2^20 (2^28 points: 1.62-1.63 ms vs 1.75 ms for the two
-// launches; 2^19 1.61 vs 1.61 ms and 2^21 1.80-1.82 vs 1.75 ms keep the two
-// launches).  NFFT_XCD=0 turns it off, NFFT_XCD=2 uses it for every length
-// that has an instance.
+// is the default: N = 2^20 (2^28 points: 1.62-1.68 ms vs 1.75 ms for the two
+// launches; 2^19 1.60 vs 1.60 ms and 2^21 1.80-1.82 vs 1.75 ms keep the two
+// launches, and so do 2^16 / 2^17 / 2^18 at 4.2 / 3.2 / 1.95 ms vs 1.50 /
+// 1.59 / 1.59 ms, where a transform has too few tiles to occupy an XCD's 32
+// workgroups with two slots).  NFFT_XCD=0 turns it off, NFFT_XCD=2 uses it
+// for every length that has an instance (N = 2^16 ... 2^21).
 // Returns 1 when the path is set up, 0 when it does not apply, -1 on an error
 // (error set).
 static int xcd_setup(C2C &c)

@@ -62,6 +62,26 @@ def test_xcd_matches_two_launch_and_oracle(pkg, dev, n, batch, row_twiddle, monk
             assert err < tol_for("f32", n)
 
 
+@pytest.mark.parametrize("n,batch", [(1 << 16, 1024), (1 << 17, 512)])
+def test_xcd_slot_reuse_small_columns(pkg, dev, n, batch, monkeypatch):
+    """N1 = 64 / 128: a transform is only 8 / 16 tiles, so each intermediate
+    slot is reused within microseconds and a CU's L1 can still hold lines of
+    the slot's previous transform when a row tile reads it.  The row tiles
+    read the slot past the L1 (agent-scope loads); plain loads after a
+    workgroup-scope invalidate returned whole stale rows here, different ones
+    on each run.  Three launches must each match the two-launch four-step
+    bit for bit."""
+    import torch
+    xd = torch.empty(2 * n * batch, dtype=torch.float32, device=dev)
+    pkg.fill_uniform(xd.data_ptr(), xd.numel(), 9, 0, "f32")
+    ref = _run(pkg, dev, n, batch, "dft", xd, False, monkeypatch)
+    for r in range(3):
+        got = _run(pkg, dev, n, batch, "dft", xd, True, monkeypatch)
+        bad = (got.view(batch, n, 2) != ref.view(batch, n, 2)).any(dim=2).any(dim=1)
+        print(f"N={n} x {batch} run {r}: transforms differing from two launches: {int(bad.sum())}")
+        assert not bool(bad.any())
+
+
 def test_xcd_in_place(pkg, dev, monkeypatch):
     import torch
     n, batch = 1 << 20, 24
