@@ -96,6 +96,8 @@ void prof_end(int slot, hipStream_t s, const char *family, int prec, int L, doub
 // launchers (nfft_k_*.hip); return hipSuccess or the launch error
 // ---------------------------------------------------------------------------
 hipError_t launch_line(int prec, int L, int dir, const LineArgs &a, hipStream_t s);
+// whether lines of L points have the half-LDS form (k_line_h)
+bool line_h_supported(int prec, int L);
 hipError_t launch_blue_line(int prec, int M, const BlueArgs &a, hipStream_t s);
 hipError_t launch_col(int prec, int L, int dir, int mode, const ColArgs &a, hipStream_t s);
 hipError_t launch_rowT(int prec, int L, int dir, const RowTArgs &a, hipStream_t s);
@@ -223,6 +225,7 @@ struct C2C {
     int prec = F32;
     int algo = ALG_LINE;
     int L = 0;           // ALG_LINE: n; ALG_BLUE1: m
+    int halves = 0;      // ALG_LINE: one line per workgroup, exchanges in two LDS halves
     int n1 = 0, n2 = 0;  // ALG_FOUR: columns FFT length n1 over n2 columns, then rows of n2
     int m = 0, m1 = 0, m2 = 0;
     const void *tw_a = nullptr, *tw_b = nullptr;  // stage tables (L or n1/m1, n2/m2)

@@ -19,6 +19,17 @@ namespace nfft {
 
 static double cxb(int prec) { return prec == F32 ? 8.0 : 16.0; }
 
+template <typename T> static bool line_h_sup(int L)
+{
+    switch (L) {
+    case 8192: return line_h_fits<T, 8192>();
+    case 16384: return line_h_fits<T, 16384>();
+    case 32768: return line_h_fits<T, 32768>();
+    default: return false;
+    }
+}
+bool line_h_supported(int prec, int L) { return prec == F32 ? line_h_sup<float>(L) : line_h_sup<double>(L); }
+
 hipError_t launch_line(int prec, int L, int dir, const LineArgs &a, hipStream_t s)
 {
     NFFT_TIMED("k_line", L, 2.0 * a.nlines * L * cxb(prec),

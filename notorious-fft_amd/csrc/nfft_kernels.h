@@ -139,6 +139,7 @@ struct LineArgs {
     long long in_dist, out_dist;  // elements between consecutive lines
     const void *tw;               // stage twiddles for L
     double scale;
+    int halves;                   // one line per workgroup, exchanges in two LDS halves (k_line_h)
 };
 
 template <typename T, int L, int DIR>
@@ -165,6 +166,137 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_line(LineArgs a
         for (int q = 0; q < S::R; q++) reg[q] = cscale(reg[q], sc);
     }
     tile_store<T, L, LPB>(reg, lds, dst, a.out_dist, L, nrows, b, j);
+}
+
+// ---------------------------------------------------------------------------
+// Lines of twice the LDS-resident length (FP32 L = 2^15, FP64 L = 2^14, 256
+// KiB): one line per workgroup of NT / 2 threads, held in VGPRs as NL = 2
+// sets of 16 points (set l of thread j: points j + l NT/2 + NT q), with each
+// Stockham exchange made through an LDS buffer of half a line in two rounds.
+// Every value a stage writes lands in a half known at compile time:
+//   - stage 0 (Ns = 1, radix R0, m = 16 / R0 >= 2 butterflies per set):
+//     butterfly i < m / 2 of either set writes points < L / 2;
+//   - a twiddled stage s < S - 1 (Ns <= L / 256, so 16 Ns divides L / 2):
+//     set 0 (j < NT / 2) writes the lower half, set 1 the upper;
+//   - the reads of q < 8 (points < 8 NT = L / 2) come from the lower half.
+// Round h writes the values of half h, then every thread reads its points of
+// that half; the lower-half reads go to a side array until the upper-half
+// values have been written.  Arithmetic per point is that of line_fft_tw.
+// ---------------------------------------------------------------------------
+template <typename T, int L> __host__ __device__ constexpr bool line_h_ok()
+{
+    using S = Sched<L>;
+    return S::S >= 2 && S::R == 16 && S::R0 <= 8 && (S::NT / 2) * 2 == S::NT && S::NT / 2 <= 1024;
+}
+
+template <typename T, int L, int DIR>
+__device__ __forceinline__ void line_fft_halves(Cx<T> (&reg)[2][16], Cx<T> *lds, int j0, const double2 *__restrict__ tw)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    static_assert(line_h_ok<T, L>(), "half-buffer line shape");
+    constexpr int H = L / 2, NH = S::NT / 2;
+    const int jj[2] = {j0, j0 + NH};
+    double2 bw[2][4];
+    sfor<0, S::S>([&](auto sc) {
+        constexpr int s = decltype(sc)::value;
+        constexpr int Rs = S::radix(s), Ns = S::ns(s), m = S::R / Rs;
+        // bases of stage s' for both sets: the same row unless ns(s') exceeds NT / 2
+        auto bases = [&](auto sn) {
+            constexpr int s2 = decltype(sn)::value;
+            constexpr bool same = NH % S::ns(s2) == 0;
+#pragma unroll
+            for (int l = 0; l < (same ? 1 : 2); l++)
+#pragma unroll
+                for (int e = 0; e < 4; e++) bw[l][e] = tw[S::twoff(s2) + (jj[l] & (S::ns(s2) - 1)) * 4 + e];
+            if constexpr (same) {
+#pragma unroll
+                for (int e = 0; e < 4; e++) bw[1][e] = bw[0][e];
+            }
+        };
+        if constexpr (s == 0) bases(std::integral_constant<int, 1>{});
+#pragma unroll
+        for (int l = 0; l < 2; l++) {
+#pragma unroll
+            for (int i = 0; i < m; i++) {
+                C a[Rs];
+#pragma unroll
+                for (int r = 0; r < Rs; r++) a[r] = reg[l][i + r * m];
+                if constexpr (s > 0) {
+                    static_assert(Rs == 16 && m == 1, "twiddled stages are radix 16");
+                    apply_stage_tw<DIR>(a, bw[l]);
+                }
+                dftR<Rs, DIR>(a);
+#pragma unroll
+                for (int r = 0; r < Rs; r++) reg[l][i + r * m] = a[r];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if constexpr (s + 1 < S::S) {
+            static_assert(s > 0 || m >= 2, "stage 0 splits by butterfly");
+            static_assert(s == 0 || (H % (16 * Ns) == 0 && NH % Ns == 0), "twiddled stages split by set");
+            if constexpr (s > 0) bases(std::integral_constant<int, s + 1>{});
+            C lo[2][8];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+#pragma unroll
+                for (int l = 0; l < 2; l++)
+#pragma unroll
+                    for (int i = 0; i < m; i++) {
+                        if ((s == 0 ? (2 * i >= m ? 1 : 0) : l) != h) continue;
+                        const int v = jj[l] + S::NT * i;
+                        const int base = (v / Ns) * Ns * Rs + (v & (Ns - 1)) - h * H;
+#pragma unroll
+                        for (int r = 0; r < Rs; r++) lds[lpad<T>(base + r * Ns)] = reg[l][i + r * m];
+                    }
+                __syncthreads();
+#pragma unroll
+                for (int l = 0; l < 2; l++)
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const C v = lds[lpad<T>(jj[l] + S::NT * q)];  // point jj + NT (q + 8 h) - h H
+                        if (h == 0) lo[l][q] = v;
+                        else reg[l][q + 8] = v;
+                    }
+                __syncthreads();
+            }
+#pragma unroll
+            for (int l = 0; l < 2; l++)
+#pragma unroll
+                for (int q = 0; q < 8; q++) reg[l][q] = lo[l][q];
+        }
+    });
+}
+
+// k_line_h: one such line per workgroup, read and written straight from
+// global memory (a wave covers 64 consecutive points per access).
+template <typename T, int L, int DIR>
+__global__ void __launch_bounds__(Sched<L>::NT / 2) k_line_h(LineArgs a)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    constexpr int NH = S::NT / 2;
+    __shared__ C lds[lpadlen<T, L / 2>()];
+    const long long line = blockIdx.x;
+    const int j = threadIdx.x;
+    const auto rs = buf_rsrc((const C *)a.in + line * a.in_dist);
+    const uint32_t off = (uint32_t)j * (uint32_t)sizeof(C);
+    C reg[2][16];
+#pragma unroll
+    for (int l = 0; l < 2; l++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) reg[l][q] = buf_ld<C>(rs, off, (uint32_t)((l * NH + S::NT * q) * sizeof(C)));
+    line_fft_halves<T, L, DIR>(reg, lds, j, (const double2 *)a.tw);
+    const T sc = (T)a.scale;
+    const auto rd = buf_rsrc((C *)a.out + line * a.out_dist);
+#pragma unroll
+    for (int l = 0; l < 2; l++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            C v = reg[l][q];
+            if (sc != (T)1) v = cscale(v, sc);
+            buf_st<C>(rd, off, (uint32_t)((l * NH + S::NT * q) * sizeof(C)), v);
+        }
 }
 
 // ---------------------------------------------------------------------------

@@ -32,9 +32,34 @@ inline void grid_trans(long long ntrans, unsigned &gy, unsigned &gz)
     if (gz < 1) gz = 1;
 }
 
+// lines of twice the LDS-resident length (k_line_h)
+template <typename T, int L> constexpr bool line_h_fits()
+{
+    return L >= 4096 && line_h_ok<T, L>() && (size_t)lpadlen<T, L / 2>() * sizeof(Cx<T>) <= kLdsLimit;
+}
+template <typename T, int L> hipError_t line_h_go(int dir, const LineArgs &a, hipStream_t s)
+{
+    if constexpr (line_h_fits<T, L>()) {
+        if (a.nlines > 0x7fffffffLL) return hipErrorInvalidValue;
+        const dim3 g((unsigned)a.nlines), b(Sched<L>::NT / 2);
+        if (dir < 0) hipLaunchKernelGGL((k_line_h<T, L, -1>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_line_h<T, L, 1>), g, b, 0, s, a);
+        return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+}
+
 // ---------------------------------------------------------------------------
 template <typename T> hipError_t line_impl(int L, int dir, const LineArgs &a, hipStream_t s)
 {
+    if (a.halves) {
+        switch (L) {
+        case 8192: return line_h_go<T, 8192>(dir, a, s);
+        case 16384: return line_h_go<T, 16384>(dir, a, s);
+        case 32768: return line_h_go<T, 32768>(dir, a, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (L) {
 #define NFFT_CASE(LL)                                                                                         \
     case LL:                                                                                                  \

@@ -604,9 +604,22 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
         set_error("transform length must be >= 1");
         return false;
     }
-    if (is_pow2(n) && n <= line_max(prec)) {
+    // Lines with their exchanges in two LDS halves (k_line_h, one line per
+    // workgroup):
+    //   - twice the LDS-resident length (FP32 2^15, FP64 2^14): one pass over
+    //     HBM instead of the four-step's two (2^28 points: FP32 1.50 -> 0.96
+    //     ms, FP64 3.04 -> 1.64 ms), unless a four-step split is asked for (the
+    //     R2C row pass that absorbs the unpack, the four-step tuner);
+    //   - 8192 points, where half the LDS lets two lines share a CU (FP32
+    //     0.786 -> 0.757 ms, FP64 1.637 -> 1.532 ms; FP32 16384 stays on the
+    //     LDS-resident line, 0.972 vs 0.997 ms).
+    // NFFT_LINE_H=0 turns both off, 2 uses the form wherever it exists.
+    const int line_h_env = env_int("NFFT_LINE_H", 1);
+    const bool line_h = n == 2LL * line_max(prec) && t_four_force.lg2 < 0 && line_h_env != 0;
+    if (is_pow2(n) && (n <= line_max(prec) || line_h)) {
         algo = ALG_LINE;
         L = n;
+        halves = line_h || (line_h_env != 0 && line_h_supported(prec, n) && (n == 8192 || line_h_env == 2)) ? 1 : 0;
         tw_a = stage_table(prec, L);
         return tw_a != nullptr;
     }
@@ -816,7 +829,9 @@ std::string C2C::describe() const
 {
     char b[128];
     switch (algo) {
-    case ALG_LINE: snprintf(b, sizeof b, "line %d", n); break;
+    case ALG_LINE:
+        snprintf(b, sizeof b, halves ? "line %d (exchanges in two LDS halves)" : "line %d", n);
+        break;
     case ALG_FOUR:
         if (pctr)
             snprintf(b, sizeof b, "four-step %dx%d persistent (tiles %d/%d, %d workgroups)", n1, n2, pcfg.lpb1,
@@ -856,7 +871,7 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
     const size_t cb = cx_bytes(prec);
     switch (algo) {
     case ALG_LINE: {
-        LineArgs a{in, out, nlines, in_dist, out_dist, tw_a, scale};
+        LineArgs a{in, out, nlines, in_dist, out_dist, tw_a, scale, halves};
         return launch_line(prec, L, dir, a, s);
     }
     case ALG_BLUE1: {

@@ -50,8 +50,11 @@ def test_pow2_single_pass(pkg, dev, prec, n):
 def test_pow2_four_step(pkg, dev, prec, n, batch, row_twiddle, monkeypatch):
     """FP32 four-steps with 1024-point rows apply the inter-pass twiddle in the
     row pass (NFFT_ROW_TWIDDLE=1, the default); 0 keeps it in the column pass
-    as every other four-step does.  Both against the oracle."""
+    as every other four-step does.  Both against the oracle.  FP32 2^15 and
+    FP64 2^14 default to one line per workgroup (test_pow2_line_halves); the
+    four-step is forced here."""
     monkeypatch.setenv("NFFT_ROW_TWIDDLE", row_twiddle)
+    monkeypatch.setenv("NFFT_LINE_H", "0")
     x = rounded(ol.uniform(2, 2 * n * batch), prec)
     for op in ("dft", "invdft"):
         got = run_plan(pkg, dev, op, [n], x, batch, prec)
@@ -59,6 +62,55 @@ def test_pow2_four_step(pkg, dev, prec, n, batch, row_twiddle, monkeypatch):
         err = ol.rel_l2(got, ref)
         print(f"{prec} {op} N={n} rel-L2 {err:.3e}")
         assert err < tol_for(prec, n), (op, err)
+
+
+@pytest.mark.parametrize("prec,n", [("f32", 1 << 15), ("f64", 1 << 14)])
+def test_pow2_line_halves(pkg, dev, prec, n):
+    """Twice the LDS-resident line length: one line per workgroup, its
+    exchanges made through half a line of LDS in two rounds (k_line_h).  A
+    batch of many workgroups, both directions, in place and out of place,
+    against the oracle."""
+    batch = 67
+    plan = pkg.Plan("dft", prec, [n], batch, "reference")
+    desc = plan.describe()
+    plan.close()
+    assert "two LDS halves" in desc, desc
+    x = rounded(ol.uniform(11, 2 * n * batch), prec)
+    for op in ("dft", "invdft"):
+        ref = ol.run_oracle(op, [n], x, batch)
+        for inplace in (False, True):
+            got = run_plan(pkg, dev, op, [n], x, batch, prec, inplace=inplace)
+            err = ol.rel_l2(got, ref)
+            print(f"{prec} {op} N={n} x{batch} in-place={inplace}: rel-L2 {err:.3e} [{desc}]")
+            assert err < tol_for(prec, n), (op, inplace, err)
+
+
+@pytest.mark.parametrize("prec,n,default_halves", [("f32", 8192, True), ("f64", 8192, True), ("f32", 16384, False)])
+def test_line_halves_bitwise(pkg, dev, monkeypatch, prec, n, default_halves):
+    """The half-LDS line and the LDS-resident line run the same arithmetic in
+    the same order per point: bit-identical outputs (NFFT_LINE_H=2 forces the
+    half form, 0 turns it off).  The default picks it at 8192 points."""
+    import torch
+    batch = 300
+    dt = torch.float32 if prec == "f32" else torch.float64
+    x = torch.empty(2 * n * batch, dtype=dt, device=dev)
+    pkg.fill_uniform(x.data_ptr(), x.numel(), 12, 0, prec)
+    plan = pkg.Plan("dft", prec, [n], batch)
+    assert ("two LDS halves" in plan.describe()) == default_halves, plan.describe()
+    plan.close()
+    outs = {}
+    for mode in ("2", "0"):
+        monkeypatch.setenv("NFFT_LINE_H", mode)
+        plan = pkg.Plan("dft", prec, [n], batch)
+        assert ("two LDS halves" in plan.describe()) == (mode == "2"), plan.describe()
+        for op in ("dft", "invdft"):
+            y = torch.empty_like(x)
+            plan.execute(op, x.data_ptr(), y.data_ptr())
+            outs[(mode, op)] = y
+        plan.check()
+        plan.close()
+    for op in ("dft", "invdft"):
+        assert torch.equal(outs[("2", op)], outs[("0", op)]), op
 
 
 @pytest.mark.parametrize("prec", ["f32", "f64"])
@@ -216,6 +268,7 @@ def test_four_step_small_scratch(pkg, dev, prec, n, batch, persist, monkeypatch)
     rotating slot (every pass-1 tile waits for the previous transform's pass
     2), the two-launch path with many chunks; in place too."""
     monkeypatch.setenv("NFFT_CHUNK_MB", "1")
+    monkeypatch.setenv("NFFT_LINE_H", "0")
     monkeypatch.setenv("NFFT_PERSIST_MB", "1")
     monkeypatch.setenv("NFFT_PERSIST", persist)
     x = rounded(ol.uniform(7, 2 * n * batch), prec)
@@ -254,6 +307,7 @@ def test_persist_many_transforms(pkg, dev, monkeypatch):
     x = rounded(ol.uniform(9, 2 * n * batch), "f32")
     ref = ol.run_oracle("dft", [n], x, batch)
     monkeypatch.setenv("NFFT_PERSIST_MB", "2")
+    monkeypatch.setenv("NFFT_LINE_H", "0")
     monkeypatch.setenv("NFFT_PERSIST", "1")
     got = run_plan(pkg, dev, "dft", [n], x, batch, "f32")
     assert ol.rel_l2(got, ref) < tol_for("f32", n)
@@ -392,6 +446,7 @@ def test_rowT_prefetch_grid_bitwise(pkg, dev, prec, lg, n2, lgtot, monkeypatch):
     x = torch.empty(2 * n * batch, dtype=dt, device=dev)
     pkg.fill_uniform(x.data_ptr(), x.numel(), 11, 0, prec)
     monkeypatch.setenv("NFFT_PERSIST", "0")
+    monkeypatch.setenv("NFFT_LINE_H", "0")  # FP32 2^15 / FP64 2^14 would run as one line per workgroup
     if n2:
         monkeypatch.setenv("NFFT_FOUR_N2", str(n2))
     outs = {}
