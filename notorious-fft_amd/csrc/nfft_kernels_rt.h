@@ -33,6 +33,7 @@ struct RtArgs {
     double scale;
     double w0;                    // weight of the DC term (vjp weightings)
     int half;                     // c2r: weight 1/2 on bins 0 < k < N/2 (realdft vjp)
+    int halves;                   // realdft / invrealdft: one line per workgroup, exchanges in two LDS halves
 };
 
 // real view of a padded complex LDS line: real index i -> slot i>>1, comp i&1
@@ -99,6 +100,110 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_r2c_line(RtArgs
         dst[k] = cscale(xk, k == 0 ? sc : sk);
         if (k == 0) dst[L] = cscale(xl, sc);
     }
+}
+
+// k_r2c_line_h / k_c2r_line_h: realdft / invrealdft of 2L reals with the
+// L-point complex line of k_line_h (one line per workgroup, NT / 2 threads
+// holding two sets of 16 points, exchanges through half a line of LDS).
+// The unpack's partner Z[L - k] of a thread's point k comes through the same
+// half buffer in two rounds: with the upper half (points of q >= 8) in LDS
+// the threads read the partners of their points 0 < k < L/2, then with the
+// lower half those of L/2 < k < L; k = 0 and k = L/2 are their own partners.
+// Each round's outputs are stored before the next round's partners are read.
+// The inverse reads both X[k] and X[L - k] from global memory, so it needs no
+// extra exchange.  Arithmetic per point is that of k_r2c_line / k_c2r_line.
+template <typename T, int L>
+__global__ void __launch_bounds__(Sched<L>::NT / 2) k_r2c_line_h(RtArgs a)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    constexpr int NH = S::NT / 2, H = L / 2;
+    __shared__ C lds[lpadlen<T, L / 2>()];
+    const long long line = blockIdx.x;
+    const int j = threadIdx.x;
+    const int jj[2] = {j, j + NH};
+    const uint32_t off = (uint32_t)j * (uint32_t)sizeof(C);
+    C reg[2][16];
+    {
+        const auto rs = buf_rsrc((const T *)a.in + line * a.in_dist);
+#pragma unroll
+        for (int l = 0; l < 2; l++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) reg[l][q] = buf_ld<C>(rs, off, (uint32_t)((l * NH + S::NT * q) * sizeof(C)));
+    }
+    line_fft_halves<T, L, -1>(reg, lds, j, (const double2 *)a.tw);
+    const C *w = (const C *)a.t1;
+    const T sc = (T)a.scale, sk = a.half ? (T)2 * sc : sc;  // half: adjoint weighting 2 on 0 < k < L
+    C *dst = (C *)a.out + line * a.out_dist;
+    // round 0: upper half in LDS, outputs of q < 8; round 1: lower half, q >= 8
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const int qs = h == 0 ? 8 : 0;  // the points written (their half is in LDS)
+        const int qo = h == 0 ? 0 : 8;  // the points whose outputs this round gives
+#pragma unroll
+        for (int l = 0; l < 2; l++)
+#pragma unroll
+            for (int q = 0; q < 8; q++) lds[lpad<T>(jj[l] + S::NT * q)] = reg[l][q + qs];  // point - (1-h) H
+        __syncthreads();
+#pragma unroll
+        for (int l = 0; l < 2; l++)
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int k = jj[l] + S::NT * (q + qo);
+                const int p = (L - k) & (L - 1);        // in the half held now unless k is 0 or L/2
+                const int pl = p - (h == 0 ? H : 0);
+                const C v = lds[lpad<T>(pl < 0 ? 0 : (pl >= H ? H - 1 : pl))];
+                const C zm = (k == 0 || k == H) ? reg[l][q + qo] : v;
+                C xk, xl;
+                unpack_pair(reg[l][q + qo], zm, w[k], xk, xl);
+                dst[k] = cscale(xk, k == 0 ? sc : sk);
+                if (k == 0) dst[L] = cscale(xl, sc);
+            }
+        if (h == 0) __syncthreads();  // the buffer is rewritten by round 1
+    }
+}
+
+template <typename T, int L>
+__global__ void __launch_bounds__(Sched<L>::NT / 2) k_c2r_line_h(RtArgs a)
+{
+    using S = Sched<L>;
+    using C = Cx<T>;
+    constexpr int NH = S::NT / 2;
+    __shared__ C lds[lpadlen<T, L / 2>()];
+    const long long line = blockIdx.x;
+    const int j = threadIdx.x;
+    const int jj[2] = {j, j + NH};
+    const C *src = (const C *)a.in + line * a.in_dist;
+    const C *w = (const C *)a.t1;
+    const T hw = a.half ? (T)0.5 : (T)1;
+    C reg[2][16];
+#pragma unroll
+    for (int l = 0; l < 2; l++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int k = jj[l] + S::NT * q;
+            C xk = src[k], xm = src[L - k];
+            if (k == 0) {  // bins 0 and N/2 are real (imaginary parts ignored)
+                xk.y = 0;
+                xm.y = 0;
+            } else {
+                xk = cscale(xk, hw);
+                xm = cscale(xm, hw);
+            }
+            reg[l][q] = pack_pair(xk, xm, w[k]);
+        }
+    line_fft_halves<T, L, 1>(reg, lds, j, (const double2 *)a.tw);
+    const T sc = (T)a.scale;
+    const auto rd = buf_rsrc((T *)a.out + line * a.out_dist);
+    const uint32_t off = (uint32_t)j * (uint32_t)sizeof(C);
+#pragma unroll
+    for (int l = 0; l < 2; l++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            C v = reg[l][q];
+            if (sc != (T)1) v = cscale(v, sc);
+            buf_st<C>(rd, off, (uint32_t)((l * NH + S::NT * q) * sizeof(C)), v);
+        }
 }
 
 template <typename T, int L>

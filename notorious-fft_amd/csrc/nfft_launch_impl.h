@@ -322,9 +322,29 @@ template <typename T> hipError_t blue_row_impl(int L, const BlueRowArgs &a, hipS
     return hipErrorInvalidValue;
 }
 
+template <typename T, int L> hipError_t r2c_h_go(bool inverse, const RtArgs &a, hipStream_t s)
+{
+    if constexpr (line_h_fits<T, L>()) {
+        if (a.nlines > 0x7fffffffLL) return hipErrorInvalidValue;
+        const dim3 g((unsigned)a.nlines), b(Sched<L>::NT / 2);
+        if (!inverse) hipLaunchKernelGGL((k_r2c_line_h<T, L>), g, b, 0, s, a);
+        else hipLaunchKernelGGL((k_c2r_line_h<T, L>), g, b, 0, s, a);
+        return hipGetLastError();
+    }
+    return hipErrorInvalidValue;
+}
+
 // ---------------------------------------------------------------------------
 template <typename T> hipError_t r2c_impl(int L, bool inverse, const RtArgs &a, hipStream_t s)
 {
+    if (a.halves) {
+        switch (L) {
+        case 8192: return r2c_h_go<T, 8192>(inverse, a, s);
+        case 16384: return r2c_h_go<T, 16384>(inverse, a, s);
+        case 32768: return r2c_h_go<T, 32768>(inverse, a, s);
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (L) {
 #define NFFT_CASE(LL)                                                                                         \
     case LL:                                                                                                  \

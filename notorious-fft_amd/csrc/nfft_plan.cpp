@@ -1136,8 +1136,18 @@ bool R2C::init(int N_, int prec_)
     L = N / 2;
     rtw = phase_table(prec, L, 1, 0, N);
     if (!rtw) return false;
-    if (L <= line_max(prec)) {
+    // complex lines of twice the LDS-resident length in the half-LDS form (one
+    // pass instead of a four-step; 2^28 reals: FP32 N = 2^16 realdft 0.88 ->
+    // 0.68 ms, invrealdft 1.44 -> 0.71 ms; FP64 2^15 1.54 -> 1.17, 2.56 ->
+    // 1.41 ms), and FP64 8192-point lines forward (N = 2^14: 1.11 -> 0.96 ms;
+    // its inverse, 1.27 vs 1.37 ms, and FP32 8192, even, keep the
+    // LDS-resident line).  NFFT_LINE_H=0 / 2 as for complex lines.
+    const int line_h_env = env_int("NFFT_LINE_H", 1);
+    const bool line_h = L == 2LL * line_max(prec) && line_h_env != 0 && line_h_supported(prec, L);
+    if (L <= line_max(prec) || line_h) {
         single = true;
+        if (line_h || (line_h_env == 2 && line_h_supported(prec, L))) halves = 3;
+        else if (line_h_env != 0 && prec == F64 && L == 8192 && line_h_supported(prec, L)) halves = 1;
         tw = stage_table(prec, L);
         return tw != nullptr;
     }
@@ -1176,7 +1186,7 @@ hipError_t R2C::fwd(const void *in, void *out, long long nlines, long long in_di
         return launch_ew(prec, EW_N1, a, 1, s);
     }
     if (single) {
-        RtArgs a{in, out, nlines, in_dist, out_dist, tw, rtw, nullptr, scale, 1.0, dbl};
+        RtArgs a{in, out, nlines, in_dist, out_dist, tw, rtw, nullptr, scale, 1.0, dbl, halves & 1};
         return launch_r2c(prec, L, a, s);
     }
     const long long chunk = scratch_elems / scratch_per_line;
@@ -1228,7 +1238,7 @@ hipError_t R2C::inv(const void *in, void *out, long long nlines, long long in_di
         return launch_ew(prec, EW_N1, a, 1, s);
     }
     if (single) {
-        RtArgs a{in, out, nlines, in_dist, out_dist, tw, rtw, nullptr, scale, 1.0, half};
+        RtArgs a{in, out, nlines, in_dist, out_dist, tw, rtw, nullptr, scale, 1.0, half, (halves >> 1) & 1};
         return launch_c2r(prec, L, a, s);
     }
     const long long chunk = scratch_elems / scratch_per_line;
@@ -1445,7 +1455,9 @@ bool nfft_plan::init(int kind_, int prec_, int d_, const int *dims_, long long b
     } else if (kind == NFFT_KIND_REALDFT) {
         if (!real.init(last, prec)) return false;
         dsc = "realdft " + std::to_string(last) +
-              (real.single ? " single-pass"
+              (real.single ? (real.halves == 3   ? " single-pass (exchanges in two LDS halves)"
+                              : real.halves == 1 ? " single-pass (forward: exchanges in two LDS halves)"
+                                                 : " single-pass")
                            : " via " + real.inner.describe() + (real.fused ? " (unpack fused into the row pass)" : ""));
         const long long H = last / 2 + 1;
         const long long Pl = P / last;  // leading product

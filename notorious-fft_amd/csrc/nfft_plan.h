@@ -15,6 +15,8 @@ hipStream_t setup_stream();
 struct R2C {
     int N = 0, prec = 0, L = 0;
     bool single = true;
+    int halves = 0;             // single: one line per workgroup, exchanges in two LDS halves
+                                // (bit 0: realdft, bit 1: invrealdft)
     const void *tw = nullptr;   // stage twiddles (single)
     const void *rtw = nullptr;  // e^{-2 pi i k/N}, k < N/2
     C2C inner;                  // multi-pass path

@@ -34,6 +34,38 @@ def test_realdft(pkg, dev, prec, n):
     assert erri < tol_for(prec, n)
 
 
+@pytest.mark.parametrize("prec,n,exact", [("f32", 16384, True), ("f64", 16384, True), ("f32", 32768, True),
+                                          ("f32", 65536, False), ("f64", 32768, False)])
+def test_realdft_line_halves_bitwise(pkg, dev, monkeypatch, prec, n, exact):
+    """The half-LDS realdft / invrealdft kernels (NFFT_LINE_H=2) against the
+    LDS-resident ones (0): the same arithmetic per point, so bit-identical
+    outputs, for both transforms and both vector-Jacobian products (the
+    adjoint weightings of bins 0 < k < N/2).  At twice the LDS-resident
+    length the alternative is the four-step (agreement within the
+    tolerance)."""
+    batch = 300 if exact else 40
+    h = n // 2 + 1
+    x = rounded(ol.uniform(31, n * batch), prec)
+    z = rounded(ol.uniform(32, 2 * h * batch), prec)
+    outs = {}
+    for mode in ("2", "0"):
+        monkeypatch.setenv("NFFT_LINE_H", mode)
+        plan = pkg.Plan("realdft", prec, [n], batch)
+        assert ("two LDS halves" in plan.describe()) == (mode == "2"), plan.describe()
+        plan.close()
+        outs[mode] = [run_plan(pkg, dev, "realdft", [n], x, batch, prec),
+                      run_plan(pkg, dev, "invrealdft", [n], z, batch, prec),
+                      run_plan(pkg, dev, "realdft", [n], z, batch, prec, vjp=True),
+                      run_plan(pkg, dev, "invrealdft", [n], x, batch, prec, vjp=True)]
+    for i, what in enumerate(("realdft", "invrealdft", "realdft vjp", "invrealdft vjp")):
+        if exact:
+            assert np.array_equal(outs["2"][i], outs["0"][i]), what
+        else:
+            assert ol.rel_l2(outs["2"][i], outs["0"][i]) < tol_for(prec, n), what
+    ref = ol.run_oracle("realdft", [n], x, batch)
+    assert ol.rel_l2(outs["2"][0], ref) < tol_for(prec, n)
+
+
 TRIG = ["dct2", "dst2", "dct3", "dst3", "dct4", "dst4"]
 T32 = [1, 2, 4, 8, 16, 128, 1024, 2048, 4096, 16384, 32768, 1 << 16]
 T64 = [1, 2, 4, 32, 1024, 2048, 16384, 1 << 15]
