@@ -277,23 +277,44 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs
         // v[i] = x[2i] (i < L), v[L+i] = sg * x[N-1-2i]; z[m] = (v[2m], v[2m+1])
         if constexpr (DIRECT) {
             // z[m] = (x[4m], x[4m+2]) for m < L/2, else (sg x[4c+3], sg x[4c+1])
-            // with c = L-1-m: two complex-pair loads of the real line each
-            const C *xl = (const C *)((const T *)a.in + (line0 + b) * a.in_dist);
+            // with c = L-1-m: two complex-pair loads of the real line each, or
+            // (FP32 lines on 16-byte boundaries) one 16-byte load
+            const T *xr = (const T *)a.in + (line0 + b) * a.in_dist;
+            const C *xl = (const C *)xr;
+            if (sizeof(T) == 4 && (((uintptr_t)xr) & 15) == 0) {  // uniform per wave (NT >= 64)
+                const float4 *x4 = (const float4 *)xr;
 #pragma unroll
-            for (int q = 0; q < S::R; q++) {
-                const int m = j + S::NT * q;
-                C v = czero<T>();
-                if (b < nrows) {
-                    if (q < S::R / 2) {
-                        const C p0 = xl[2 * m], p1 = xl[2 * m + 1];
-                        v = cmk<C>(p0.x, p1.x);
-                    } else {
-                        const int c = L - 1 - m;
-                        const C p0 = xl[2 * c], p1 = xl[2 * c + 1];
-                        v = cmk<C>(sg * p1.y, sg * p0.y);
+                for (int q = 0; q < S::R; q++) {
+                    const int m = j + S::NT * q;
+                    C v = czero<T>();
+                    if (b < nrows) {
+                        if (q < S::R / 2) {
+                            const float4 p = x4[m];
+                            v = cmk<C>((T)p.x, (T)p.z);
+                        } else {
+                            const float4 p = x4[L - 1 - m];
+                            v = cmk<C>(sg * (T)p.w, sg * (T)p.y);
+                        }
                     }
+                    reg[q] = v;
                 }
-                reg[q] = v;
+            } else {
+#pragma unroll
+                for (int q = 0; q < S::R; q++) {
+                    const int m = j + S::NT * q;
+                    C v = czero<T>();
+                    if (b < nrows) {
+                        if (q < S::R / 2) {
+                            const C p0 = xl[2 * m], p1 = xl[2 * m + 1];
+                            v = cmk<C>(p0.x, p1.x);
+                        } else {
+                            const int c = L - 1 - m;
+                            const C p0 = xl[2 * c], p1 = xl[2 * c + 1];
+                            v = cmk<C>(sg * p1.y, sg * p0.y);
+                        }
+                    }
+                    reg[q] = v;
+                }
             }
         } else {
             auto xv = [&](int i) -> T {
@@ -314,20 +335,25 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs
         // outputs for k = j + NT q: (y[k], y[N-k]) (DCT2) from X[k]; X[0] and X[L] special
         const C *rw = (const C *)a.t2;   // e^{-2 pi i k/N}
         const C *dw = (const C *)a.t1;   // e^{-i pi k/(2N)}
+        // the tables at k = j + NT q as w(j) w(NT q): two per-lane loads and
+        // 2 (R - 1) workgroup-uniform ones instead of 2 R per-lane loads
+        const C rwj = rw[j], dwj = dw[j];
         T ya[S::R], yb[S::R];
         T yl = 0;
 #pragma unroll
         for (int q = 0; q < S::R; q++) {
             const int k = j + S::NT * q;
             const C zk = ln[lpad<T>(k)], zm = ln[lpad<T>((L - k) & (L - 1))];
+            const C rwk = q == 0 ? rwj : cmul(rwj, rw[S::NT * q]);
             C xk, xl;
-            unpack_pair(zk, zm, rw[k], xk, xl);
+            unpack_pair(zk, zm, rwk, xk, xl);
             if (k == 0) {
                 ya[q] = (T)2 * xk.x * (T)a.w0;
                 yb[q] = 0;
                 yl = sq2 * xl.x;
             } else {
-                const C p = cmul(xk, dw[k]);
+                const C dwk = q == 0 ? dwj : cmul(dwj, dw[S::NT * q]);
+                const C p = cmul(xk, dwk);
                 ya[q] = (T)2 * p.x;
                 yb[q] = (T)-2 * p.y;
             }
@@ -396,7 +422,8 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs
             if constexpr (DIRECT) return xg[i];
             else return lds_real<T, L>(line, i);
         };
-        auto X = [&](int k) -> C {
+        // X(k) with its twiddle w = dw[k] (unused at k = 0 and k = L)
+        auto X = [&](int k, C w) -> C {
             if (k == 0) {
                 const T v = OP == T_DCT3 ? rx(0) : rx(N - 1);
                 return cmk<C>(v * (T)a.w0, (T)0);
@@ -413,12 +440,17 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs
                 xr = rx(N - 1 - k);
                 xi = rx(k - 1);
             }
-            return cconj(cmul(cmk<C>(xr, xi), dw[k]));
+            return cconj(cmul(cmk<C>(xr, xi), w));
         };
+        // tables at k = j + NT q as w(j) w(NT q), and dw[L - k] = e^{-i pi/4} conj(dw[k])
+        const C rwj = rw[j], dwj = dw[j];
+        const C e8 = cmk<C>((T)0.70710678118654752440, (T)-0.70710678118654752440);
 #pragma unroll
         for (int q = 0; q < S::R; q++) {
             const int k = j + S::NT * q;
-            reg[q] = pack_pair(X(k), X(L - k), rw[k]);
+            const C dwk = q == 0 ? dwj : cmul(dwj, dw[S::NT * q]);
+            const C rwk = q == 0 ? rwj : cmul(rwj, rw[S::NT * q]);
+            reg[q] = pack_pair(X(k, dwk), X(L - k, cmul(e8, cconj(dwk))), rwk);
         }
         if constexpr (!DIRECT) __syncthreads();
         line_fft<T, L, 1>(reg, lds, lay, j, (const double2 *)a.tw);
@@ -457,6 +489,10 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t4_line(RtArgs 
     const C *src = (const C *)((const T *)a.in + line0 * a.in_dist);
     const C *pre = (const C *)a.t1;
     const C *post = (const C *)a.t2;
+    // tables at n = j + NT q as w(j) w(NT q): pre[n] = e^{-i pi n/N}, and
+    // post[2n + e] = post[2j + e] pre[NT q]
+    const C prej = pre[j];
+    auto pre_at = [&](int q) -> C { return q == 0 ? prej : cmul(prej, pre[S::NT * q]); };
     C reg[S::R];
     if constexpr (S::NT >= 64) {
         // x[2n] = pair n .x, x[N-1-2n] = pair L-1-n .y, straight from global memory
@@ -469,8 +505,8 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t4_line(RtArgs 
                 xe = xl[n].x;
                 xo = xl[L - 1 - n].y;
             }
-            if constexpr (OP == T_DCT4) reg[q] = cmul(pre[n], cmk<C>(xe, xo));
-            else reg[q] = cneg(cmul(pre[n], cmk<C>(xe, -xo)));
+            if constexpr (OP == T_DCT4) reg[q] = cmul(pre_at(q), cmk<C>(xe, xo));
+            else reg[q] = cneg(cmul(pre_at(q), cmk<C>(xe, -xo)));
         }
     } else {
         stage_in_rows<T, L, LPB, NTH>(lds, src, a.in_dist / 2, L, nrows);
@@ -480,8 +516,8 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t4_line(RtArgs 
         for (int q = 0; q < S::R; q++) {
             const int n = j + S::NT * q;
             const T xe = lds_real<T, L>(line, 2 * n), xo = lds_real<T, L>(line, N - 1 - 2 * n);
-            if constexpr (OP == T_DCT4) reg[q] = cmul(pre[n], cmk<C>(xe, xo));
-            else reg[q] = cneg(cmul(pre[n], cmk<C>(xe, -xo)));
+            if constexpr (OP == T_DCT4) reg[q] = cmul(pre_at(q), cmk<C>(xe, xo));
+            else reg[q] = cneg(cmul(pre_at(q), cmk<C>(xe, -xo)));
         }
         __syncthreads();
     }
@@ -492,11 +528,14 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t4_line(RtArgs 
     for (int q = 0; q < S::R; q++) ln[lpad<T>(j + S::NT * q)] = reg[q];
     __syncthreads();
     const T sc = (T)a.scale;
+    const C post0 = post[2 * j], post1 = post[2 * j + 1];
 #pragma unroll
     for (int q = 0; q < S::R; q++) {
         const int k = j + S::NT * q;
         const C tk = ln[lpad<T>(k)], tm = cconj(ln[lpad<T>(L - 1 - k)]);
-        const C p0 = cmul(post[2 * k], tk), p1 = cmul(post[2 * k + 1], tm);
+        const C s = pre[S::NT * q];  // workgroup-uniform
+        const C w0 = q == 0 ? post0 : cmul(post0, s), w1 = q == 0 ? post1 : cmul(post1, s);
+        const C p0 = cmul(w0, tk), p1 = cmul(w1, tm);
         if constexpr (OP == T_DCT4) reg[q] = cmk<C>((T)2 * p0.x * sc, (T)2 * p1.x * sc);
         else reg[q] = cmk<C>((T)2 * p0.y * sc, (T)2 * p1.y * sc);
     }
