@@ -90,13 +90,14 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_r2c_line(RtArgs
     if (b >= nrows) return;
     C *dst = (C *)a.out + (line0 + b) * a.out_dist;
     const C *w = (const C *)a.t1;
+    const C wj = w[j];  // w[k] = w[j] w[NT q] (the second workgroup-uniform)
     const T sc = (T)a.scale, sk = a.half ? (T)2 * sc : sc;  // half: adjoint weighting 2 on 0 < k < L
 #pragma unroll
     for (int q = 0; q < S::R; q++) {
         const int k = j + S::NT * q;
         const C zk = line[lpad<T>(k)], zm = line[lpad<T>((L - k) & (L - 1))];
         C xk, xl;
-        unpack_pair(zk, zm, w[k], xk, xl);
+        unpack_pair(zk, zm, q == 0 ? wj : cmul(wj, w[S::NT * q]), xk, xl);
         dst[k] = cscale(xk, k == 0 ? sc : sk);
         if (k == 0) dst[L] = cscale(xl, sc);
     }
@@ -133,6 +134,7 @@ __global__ void __launch_bounds__(Sched<L>::NT / 2) k_r2c_line_h(RtArgs a)
     }
     line_fft_halves<T, L, -1>(reg, lds, j, (const double2 *)a.tw);
     const C *w = (const C *)a.t1;
+    const C wj[2] = {w[jj[0]], w[jj[1]]};  // w[k] = w[jj] w[NT q] (the second workgroup-uniform)
     const T sc = (T)a.scale, sk = a.half ? (T)2 * sc : sc;  // half: adjoint weighting 2 on 0 < k < L
     C *dst = (C *)a.out + line * a.out_dist;
     // round 0: upper half in LDS, outputs of q < 8; round 1: lower half, q >= 8
@@ -155,7 +157,7 @@ __global__ void __launch_bounds__(Sched<L>::NT / 2) k_r2c_line_h(RtArgs a)
                 const C v = lds[lpad<T>(pl < 0 ? 0 : (pl >= H ? H - 1 : pl))];
                 const C zm = (k == 0 || k == H) ? reg[l][q + qo] : v;
                 C xk, xl;
-                unpack_pair(reg[l][q + qo], zm, w[k], xk, xl);
+                unpack_pair(reg[l][q + qo], zm, q + qo == 0 ? wj[l] : cmul(wj[l], w[S::NT * (q + qo)]), xk, xl);
                 dst[k] = cscale(xk, k == 0 ? sc : sk);
                 if (k == 0) dst[L] = cscale(xl, sc);
             }
@@ -175,6 +177,7 @@ __global__ void __launch_bounds__(Sched<L>::NT / 2) k_c2r_line_h(RtArgs a)
     const int jj[2] = {j, j + NH};
     const C *src = (const C *)a.in + line * a.in_dist;
     const C *w = (const C *)a.t1;
+    const C wj[2] = {w[jj[0]], w[jj[1]]};  // w[k] = w[jj] w[NT q] (the second workgroup-uniform)
     const T hw = a.half ? (T)0.5 : (T)1;
     C reg[2][16];
 #pragma unroll
@@ -190,7 +193,7 @@ __global__ void __launch_bounds__(Sched<L>::NT / 2) k_c2r_line_h(RtArgs a)
                 xk = cscale(xk, hw);
                 xm = cscale(xm, hw);
             }
-            reg[l][q] = pack_pair(xk, xm, w[k]);
+            reg[l][q] = pack_pair(xk, xm, q == 0 ? wj[l] : cmul(wj[l], w[S::NT * q]));
         }
     line_fft_halves<T, L, 1>(reg, lds, j, (const double2 *)a.tw);
     const T sc = (T)a.scale;
@@ -219,6 +222,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_c2r_line(RtArgs
     const int nrows = rem < LPB ? (int)rem : LPB;
     const C *src = (const C *)a.in + (line0 + (b < nrows ? b : 0)) * a.in_dist;
     const C *w = (const C *)a.t1;
+    const C wj = w[j];  // w[k] = w[j] w[NT q] (the second workgroup-uniform)
     C reg[S::R];
     const T hw = a.half ? (T)0.5 : (T)1;
 #pragma unroll
@@ -232,7 +236,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_c2r_line(RtArgs
             xk = cscale(xk, hw);
             xm = cscale(xm, hw);
         }
-        reg[q] = pack_pair(xk, xm, w[k]);
+        reg[q] = pack_pair(xk, xm, q == 0 ? wj : cmul(wj, w[S::NT * q]));
     }
     LayLine<T, L> lay{b * lpadlen<T, L>()};
     line_fft<T, L, 1>(reg, lds, lay, j, (const double2 *)a.tw);
