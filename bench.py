@@ -128,10 +128,20 @@ def run_r2c(pkg, torch, n, prec, total_reals, steps=5, warmup=2):
     return out
 
 
-def timed(torch, fn, steps=5, warmup=2):
+def timed(torch, fn, steps=5, warmup=2, warm_s=0.05):
+    """Seconds per call of fn over `steps` calls, after at least `warmup`
+    calls and at least warm_s seconds of them: after host-side work (the CPU
+    baselines and oracle checks run between configurations) the GPU needs
+    tens of milliseconds of load to return to full clocks, and a fixed count
+    of sub-millisecond calls measured up to 20 % slow."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
+    t_end = time.perf_counter() + warm_s
+    while time.perf_counter() < t_end:
+        for _ in range(4):
+            fn()
+        torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
         fn()
@@ -282,9 +292,8 @@ def config_extras(pkg, torch, only=None, with_cpu=True):
         p = pkg.Plan(kind, prec, dims, batch)
         fn = (lambda: p.vjp(op, x.data_ptr(), y.data_ptr())) if vjp else \
             (lambda: p.execute(op, x.data_ptr(), y.data_ptr()))
-        # 10 warmup calls: after host-side work (the oracle checks before
-        # these configurations) the GPU needs several milliseconds of load to
-        # return to full clocks; with 2 the timed calls ran up to 2x slow
+        # 10 warmup calls and at least 50 ms of them (timed): after host-side
+        # work the GPU needs load to return to full clocks
         t = timed(torch, fn, steps=20, warmup=10)
         eb = 4 if prec == "f32" else 8
         out[name] = {"ms": round(1e3 * t, 4), "gflops": round(flops * batch / t / 1e9, 1),
