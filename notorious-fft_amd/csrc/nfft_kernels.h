@@ -189,6 +189,18 @@ template <typename T, int L> __host__ __device__ constexpr bool line_h_ok()
     return S::S >= 2 && S::R == 16 && S::R0 <= 8 && (S::NT / 2) * 2 == S::NT && S::NT / 2 <= 1024;
 }
 
+// Waves per SIMD the FP32 half-LDS kernels are compiled for: as many as the
+// workgroups that fit in the LDS can use, capped at 4 (<= 128 VGPRs; at 131
+// the 16384-point line held one workgroup per CU instead of two).
+template <typename T, int L> __host__ __device__ constexpr int line_h_wpe()
+{
+    if (sizeof(T) != 4) return 1;
+    const long lds = (long)lpadlen<T, L / 2>() * (long)sizeof(Cx<T>);
+    const long wg = (160L * 1024) / lds;
+    const long w = wg * (Sched<L>::NT / 2 / 64) / 4;
+    return w < 1 ? 1 : (w > 4 ? 4 : (int)w);
+}
+
 template <typename T, int L, int DIR>
 __device__ __forceinline__ void line_fft_halves(Cx<T> (&reg)[2][16], Cx<T> *lds, int j0, const double2 *__restrict__ tw)
 {
@@ -271,7 +283,7 @@ __device__ __forceinline__ void line_fft_halves(Cx<T> (&reg)[2][16], Cx<T> *lds,
 // k_line_h: one such line per workgroup, read and written straight from
 // global memory (a wave covers 64 consecutive points per access).
 template <typename T, int L, int DIR>
-__global__ void __launch_bounds__(Sched<L>::NT / 2) k_line_h(LineArgs a)
+__global__ void __launch_bounds__(Sched<L>::NT / 2, (line_h_wpe<T, L>())) k_line_h(LineArgs a)
 {
     using S = Sched<L>;
     using C = Cx<T>;

@@ -114,7 +114,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_r2c_line(RtArgs
 // The inverse reads both X[k] and X[L - k] from global memory, so it needs no
 // extra exchange.  Arithmetic per point is that of k_r2c_line / k_c2r_line.
 template <typename T, int L>
-__global__ void __launch_bounds__(Sched<L>::NT / 2) k_r2c_line_h(RtArgs a)
+__global__ void __launch_bounds__(Sched<L>::NT / 2, (line_h_wpe<T, L>())) k_r2c_line_h(RtArgs a)
 {
     using S = Sched<L>;
     using C = Cx<T>;
@@ -166,7 +166,7 @@ __global__ void __launch_bounds__(Sched<L>::NT / 2) k_r2c_line_h(RtArgs a)
 }
 
 template <typename T, int L>
-__global__ void __launch_bounds__(Sched<L>::NT / 2) k_c2r_line_h(RtArgs a)
+__global__ void __launch_bounds__(Sched<L>::NT / 2, (line_h_wpe<T, L>())) k_c2r_line_h(RtArgs a)
 {
     using S = Sched<L>;
     using C = Cx<T>;

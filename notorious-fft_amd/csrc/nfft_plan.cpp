@@ -610,16 +610,17 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
     //     HBM instead of the four-step's two (2^28 points: FP32 1.50 -> 0.96
     //     ms, FP64 3.04 -> 1.64 ms), unless a four-step split is asked for (the
     //     R2C row pass that absorbs the unpack, the four-step tuner);
-    //   - 8192 points, where half the LDS lets two lines share a CU (FP32
-    //     0.786 -> 0.757 ms, FP64 1.637 -> 1.532 ms; FP32 16384 stays on the
-    //     LDS-resident line, 0.972 vs 0.997 ms).
-    // NFFT_LINE_H=0 turns both off, 2 uses the form wherever it exists.
+    //   - 8192 points (FP32 0.786 -> 0.757-0.773 ms, FP64 1.637 -> 1.532 ms)
+    //     and FP32 16384 (0.977 -> 0.808 ms), where half the LDS lets two or
+    //     more lines share a CU.
+    // These are all the lengths the form exists for.  NFFT_LINE_H=0 turns it
+    // off.
     const int line_h_env = env_int("NFFT_LINE_H", 1);
     const bool line_h = n == 2LL * line_max(prec) && t_four_force.lg2 < 0 && line_h_env != 0;
     if (is_pow2(n) && (n <= line_max(prec) || line_h)) {
         algo = ALG_LINE;
         L = n;
-        halves = line_h || (line_h_env != 0 && line_h_supported(prec, n) && (n == 8192 || line_h_env == 2)) ? 1 : 0;
+        halves = line_h || (line_h_env != 0 && line_h_supported(prec, n)) ? 1 : 0;
         tw_a = stage_table(prec, L);
         return tw_a != nullptr;
     }
@@ -1138,15 +1139,18 @@ bool R2C::init(int N_, int prec_)
     if (!rtw) return false;
     // complex lines of twice the LDS-resident length in the half-LDS form (one
     // pass instead of a four-step; 2^28 reals: FP32 N = 2^16 realdft 0.88 ->
-    // 0.68 ms, invrealdft 1.44 -> 0.71 ms; FP64 2^15 1.54 -> 1.17, 2.56 ->
-    // 1.41 ms), and FP64 8192-point lines forward (N = 2^14: 1.11 -> 0.96 ms;
-    // its inverse, 1.27 vs 1.37 ms, and FP32 8192, even, keep the
+    // 0.61 ms, invrealdft 1.44 -> 0.65 ms; FP64 2^15 1.54 -> 1.06, 2.56 ->
+    // 1.16 ms), FP32 16384-point lines (N = 2^15: 0.58 -> 0.55 ms, inverse
+    // 0.65 -> 0.48 ms) and FP64 8192-point lines forward (N = 2^14: 1.06 ->
+    // 0.91 ms; its inverse, 1.10 both ways, and FP32 8192, even, keep the
     // LDS-resident line).  NFFT_LINE_H=0 / 2 as for complex lines.
     const int line_h_env = env_int("NFFT_LINE_H", 1);
     const bool line_h = L == 2LL * line_max(prec) && line_h_env != 0 && line_h_supported(prec, L);
     if (L <= line_max(prec) || line_h) {
         single = true;
-        if (line_h || (line_h_env == 2 && line_h_supported(prec, L))) halves = 3;
+        if (line_h || (line_h_env == 2 && line_h_supported(prec, L)) ||
+            (line_h_env != 0 && prec == F32 && L == 16384 && line_h_supported(prec, L)))
+            halves = 3;
         else if (line_h_env != 0 && prec == F64 && L == 8192 && line_h_supported(prec, L)) halves = 1;
         tw = stage_table(prec, L);
         return tw != nullptr;

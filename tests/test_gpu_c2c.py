@@ -85,11 +85,13 @@ def test_pow2_line_halves(pkg, dev, prec, n):
             assert err < tol_for(prec, n), (op, inplace, err)
 
 
-@pytest.mark.parametrize("prec,n,default_halves", [("f32", 8192, True), ("f64", 8192, True), ("f32", 16384, False)])
+@pytest.mark.parametrize("prec,n,default_halves", [("f32", 8192, True), ("f64", 8192, True), ("f32", 16384, True),
+                                                    ("f32", 4096, False)])
 def test_line_halves_bitwise(pkg, dev, monkeypatch, prec, n, default_halves):
     """The half-LDS line and the LDS-resident line run the same arithmetic in
-    the same order per point: bit-identical outputs (NFFT_LINE_H=2 forces the
-    half form, 0 turns it off).  The default picks it at 8192 points."""
+    the same order per point: bit-identical outputs (NFFT_LINE_H=0 turns the
+    half form off).  The default picks it wherever it exists (from 8192
+    points; 4096 has no such form)."""
     import torch
     batch = 300
     dt = torch.float32 if prec == "f32" else torch.float64
@@ -102,7 +104,7 @@ def test_line_halves_bitwise(pkg, dev, monkeypatch, prec, n, default_halves):
     for mode in ("2", "0"):
         monkeypatch.setenv("NFFT_LINE_H", mode)
         plan = pkg.Plan("dft", prec, [n], batch)
-        assert ("two LDS halves" in plan.describe()) == (mode == "2"), plan.describe()
+        assert ("two LDS halves" in plan.describe()) == (mode == "2" and default_halves), plan.describe()
         for op in ("dft", "invdft"):
             y = torch.empty_like(x)
             plan.execute(op, x.data_ptr(), y.data_ptr())
