@@ -36,6 +36,14 @@ struct RtArgs {
     int halves;                   // realdft / invrealdft: one line per workgroup, exchanges in two LDS halves
 };
 
+// Waves per SIMD the FP32 single-pass real / trig line kernels are compiled
+// for (wpe_hint: as many as their LDS allows, at most 4, i.e. <= 128 VGPRs):
+// above 128 VGPRs an 8192-point line kernel held one workgroup per CU.
+template <typename T, int L> __host__ __device__ constexpr int rt_wpe()
+{
+    return wpe_hint<T>(Sched<L>::NT * line_lpb<L>(), (long)line_lds<T, L, line_lpb<L>()>() * (long)sizeof(Cx<T>));
+}
+
 // real view of a padded complex LDS line: real index i -> slot i>>1, comp i&1
 template <typename T, int L> __device__ __forceinline__ T lds_real(const Cx<T> *line, int i)
 {
@@ -68,7 +76,7 @@ template <typename C> __device__ __forceinline__ C pack_pair(C xk, C xm, C w)
 
 // ---------------------------------------------------------------------------
 template <typename T, int L>
-__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_r2c_line(RtArgs a)
+__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>(), (rt_wpe<T, L>())) k_r2c_line(RtArgs a)
 {
     using S = Sched<L>;
     using C = Cx<T>;
@@ -210,7 +218,7 @@ __global__ void __launch_bounds__(Sched<L>::NT / 2, (line_h_wpe<T, L>())) k_c2r_
 }
 
 template <typename T, int L>
-__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_c2r_line(RtArgs a)
+__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>(), (rt_wpe<T, L>())) k_c2r_line(RtArgs a)
 {
     using S = Sched<L>;
     using C = Cx<T>;
@@ -250,8 +258,9 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_c2r_line(RtArgs
 }
 
 // DCT/DST types 2 and 3 -----------------------------------------------------
+// (DST-3 keeps the compiler's register choice: at 128 VGPRs it spills)
 template <typename T, int L, int OP>
-__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs a)
+__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>(), (OP == T_DST3 ? 1 : rt_wpe<T, L>())) k_t23_line(RtArgs a)
 {
     using S = Sched<L>;
     using C = Cx<T>;
@@ -478,7 +487,7 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t23_line(RtArgs
 
 // DCT/DST type 4 -------------------------------------------------------------
 template <typename T, int L, int OP>
-__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_t4_line(RtArgs a)
+__global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>(), (rt_wpe<T, L>())) k_t4_line(RtArgs a)
 {
     using S = Sched<L>;
     using C = Cx<T>;
