@@ -48,9 +48,10 @@ class NativeError(RuntimeError):
     pass
 
 
-def build(jobs=8, verbose=False):
-    """Compile every HIP source for gfx950 into this directory (make)."""
-    cmd = ["make", "-C", CSRC, f"-j{jobs}"]
+def build(jobs=8, verbose=False, force=True):
+    """Compile every HIP source for gfx950 into this directory (make; with
+    force, every target is rebuilt even where it is up to date)."""
+    cmd = ["make", "-C", CSRC, f"-j{jobs}"] + (["-B"] if force else [])
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     if verbose or r.returncode:
         print(r.stdout[-4000:])
