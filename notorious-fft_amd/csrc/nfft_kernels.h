@@ -169,17 +169,19 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>()) k_line(LineArgs a
 }
 
 // ---------------------------------------------------------------------------
-// Lines of twice the LDS-resident length (FP32 L = 2^15, FP64 L = 2^14, 256
-// KiB): one line per workgroup of NT / 2 threads, held in VGPRs as NL = 2
-// sets of 16 points (set l of thread j: points j + l NT/2 + NT q), with each
-// Stockham exchange made through an LDS buffer of half a line in two rounds.
+// Lines with their exchanges through half a line of LDS: twice the
+// LDS-resident length (FP32 L = 2^15, FP64 L = 2^14: 256 KiB), and 8192 /
+// FP32 16384 points, where the smaller buffer lets more lines share a CU.
+// One line per workgroup of NT / 2 threads, held in VGPRs as NL = 2 sets of
+// 16 points (set l of thread j: points j + l NT/2 + NT q), with each
+// Stockham exchange made through an LDS buffer of half a line in two steps.
 // Every value a stage writes lands in a half known at compile time:
 //   - stage 0 (Ns = 1, radix R0, m = 16 / R0 >= 2 butterflies per set):
 //     butterfly i < m / 2 of either set writes points < L / 2;
 //   - a twiddled stage s < S - 1 (Ns <= L / 256, so 16 Ns divides L / 2):
 //     set 0 (j < NT / 2) writes the lower half, set 1 the upper;
 //   - the reads of q < 8 (points < 8 NT = L / 2) come from the lower half.
-// Round h writes the values of half h, then every thread reads its points of
+// Step h writes the values of half h, then every thread reads its points of
 // that half; the lower-half reads go to a side array until the upper-half
 // values have been written.  Arithmetic per point is that of line_fft_tw.
 // ---------------------------------------------------------------------------

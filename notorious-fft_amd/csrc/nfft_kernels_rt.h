@@ -115,10 +115,10 @@ __global__ void __launch_bounds__(Sched<L>::NT *line_lpb<L>(), (rt_wpe<T, L>()))
 // L-point complex line of k_line_h (one line per workgroup, NT / 2 threads
 // holding two sets of 16 points, exchanges through half a line of LDS).
 // The unpack's partner Z[L - k] of a thread's point k comes through the same
-// half buffer in two rounds: with the upper half (points of q >= 8) in LDS
+// half buffer in two steps: with the upper half (points of q >= 8) in LDS
 // the threads read the partners of their points 0 < k < L/2, then with the
 // lower half those of L/2 < k < L; k = 0 and k = L/2 are their own partners.
-// Each round's outputs are stored before the next round's partners are read.
+// Each step's outputs are stored before the next step's partners are read.
 // The inverse reads both X[k] and X[L - k] from global memory, so it needs no
 // extra exchange.  Arithmetic per point is that of k_r2c_line / k_c2r_line.
 template <typename T, int L>
@@ -145,11 +145,11 @@ __global__ void __launch_bounds__(Sched<L>::NT / 2, (line_h_wpe<T, L>())) k_r2c_
     const C wj[2] = {w[jj[0]], w[jj[1]]};  // w[k] = w[jj] w[NT q] (the second workgroup-uniform)
     const T sc = (T)a.scale, sk = a.half ? (T)2 * sc : sc;  // half: adjoint weighting 2 on 0 < k < L
     C *dst = (C *)a.out + line * a.out_dist;
-    // round 0: upper half in LDS, outputs of q < 8; round 1: lower half, q >= 8
+    // h = 0: upper half in LDS, outputs of q < 8; h = 1: lower half, q >= 8
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         const int qs = h == 0 ? 8 : 0;  // the points written (their half is in LDS)
-        const int qo = h == 0 ? 0 : 8;  // the points whose outputs this round gives
+        const int qo = h == 0 ? 0 : 8;  // the points whose outputs this step gives
 #pragma unroll
         for (int l = 0; l < 2; l++)
 #pragma unroll
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(Sched<L>::NT / 2, (line_h_wpe<T, L>())) k_r2c_
                 dst[k] = cscale(xk, k == 0 ? sc : sk);
                 if (k == 0) dst[L] = cscale(xl, sc);
             }
-        if (h == 0) __syncthreads();  // the buffer is rewritten by round 1
+        if (h == 0) __syncthreads();  // the buffer is rewritten for h = 1
     }
 }
 
