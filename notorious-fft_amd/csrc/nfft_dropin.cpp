@@ -19,6 +19,8 @@
 // to N/2+1 complex bins.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <cstring>
 #include <new>
 #include <vector>
 
@@ -46,7 +48,24 @@ struct Impl {
     hipStream_t st = nullptr;
     void *din = nullptr, *dout = nullptr;
     size_t stage_bytes = 0;
+    void *hin = nullptr, *hout = nullptr;  // pinned host staging of small host-buffer calls
+    void *hin_d = nullptr, *hout_d = nullptr;  // their device-side addresses
+    size_t hstage_bytes = 0;
 };
+
+// Host buffers up to this size go through pinned staging buffers of the aux
+// (a host copy into and out of them, DMA transfers on the aux's stream):
+// lower latency than transfers from pageable memory for small transforms,
+// while larger ones keep the direct pageable transfer (no extra host copy).
+constexpr size_t kPinnedStageMax = 4u << 20;
+
+// Up to this many bytes a 1-D transform runs on the mapped staging buffers
+// directly, without DMA (NFFT_DROPIN_ZC_MAX overrides; 0 turns it off).
+size_t zero_copy_max()
+{
+    const char *e = getenv("NFFT_DROPIN_ZC_MAX");
+    return e && *e ? (size_t)strtoull(e, nullptr, 10) : (size_t)(256u << 10);
+}
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
@@ -128,6 +147,41 @@ void run(const notorious_fft_aux *a, int op, const void *x, void *y)
         return;
     }
     const size_t need = ib > ob ? ib : ob;
+    bool pin = need <= kPinnedStageMax;
+    if (pin && im->hstage_bytes < need) {
+        if (im->hin) (void)hipHostFree(im->hin);
+        if (im->hout) (void)hipHostFree(im->hout);
+        im->hin = im->hout = im->hin_d = im->hout_d = nullptr;
+        im->hstage_bytes = 0;
+        const unsigned fl = hipHostMallocMapped | hipHostMallocCoherent;
+        if (hipHostMalloc(&im->hin, need, fl) == hipSuccess && hipHostMalloc(&im->hout, need, fl) == hipSuccess &&
+            hipHostGetDevicePointer(&im->hin_d, im->hin, 0) == hipSuccess &&
+            hipHostGetDevicePointer(&im->hout_d, im->hout, 0) == hipSuccess) {
+            im->hstage_bytes = need;
+        } else {
+            (void)hipGetLastError();
+            if (im->hin) (void)hipHostFree(im->hin);
+            if (im->hout) (void)hipHostFree(im->hout);
+            im->hin = im->hout = im->hin_d = im->hout_d = nullptr;
+            pin = false;  // pageable transfers instead
+        }
+    }
+    if (pin && need <= zero_copy_max() && im->dims.size() == 1) {
+        // The transform reads and writes the mapped staging buffers in place
+        // over the bus: no DMA transfers, whose fixed cost (about 10 us each
+        // for 64 KiB) exceeds a one-pass transform at these sizes.  N-D plans
+        // make several passes over their output and keep the DMA transfers
+        // (each pass over the bus would cost more than the two copies).
+        const void *src = x;
+        if (!xd) {
+            memcpy(im->hin, x, ib);
+            src = im->hin_d;
+        }
+        if (nfft_execute(im->plan, op, src, yd ? y : im->hout_d, im->st) != NFFT_OK) return;
+        if (hipStreamSynchronize(im->st) != hipSuccess) return;
+        if (!yd) memcpy(y, im->hout, ob);
+        return;
+    }
     if (im->stage_bytes < need) {
         if (im->din) (void)hipFree(im->din);
         if (im->dout) (void)hipFree(im->dout);
@@ -140,11 +194,24 @@ void run(const notorious_fft_aux *a, int op, const void *x, void *y)
     const void *src = x;
     void *dst = yd ? y : im->dout;
     if (!xd) {
-        if (hipMemcpyAsync(im->din, x, ib, hipMemcpyHostToDevice, im->st) != hipSuccess) return;
+        if (pin) {
+            memcpy(im->hin, x, ib);
+            if (hipMemcpyAsync(im->din, im->hin, ib, hipMemcpyHostToDevice, im->st) != hipSuccess) return;
+        } else if (hipMemcpyAsync(im->din, x, ib, hipMemcpyHostToDevice, im->st) != hipSuccess) {
+            return;
+        }
         src = im->din;
     }
     if (nfft_execute(im->plan, op, src, dst, im->st) != NFFT_OK) return;
-    if (!yd) (void)hipMemcpyAsync(y, im->dout, ob, hipMemcpyDeviceToHost, im->st);
+    if (!yd) {
+        if (pin) {
+            if (hipMemcpyAsync(im->hout, im->dout, ob, hipMemcpyDeviceToHost, im->st) != hipSuccess) return;
+            if (hipStreamSynchronize(im->st) != hipSuccess) return;
+            memcpy(y, im->hout, ob);
+            return;
+        }
+        (void)hipMemcpyAsync(y, im->dout, ob, hipMemcpyDeviceToHost, im->st);
+    }
     (void)hipStreamSynchronize(im->st);
 }
 
@@ -254,6 +321,8 @@ void notorious_fft_free_aux(notorious_fft_aux *a)
         nfft_plan_destroy(im->plan);
         if (im->din) (void)hipFree(im->din);
         if (im->dout) (void)hipFree(im->dout);
+        if (im->hin) (void)hipHostFree(im->hin);
+        if (im->hout) (void)hipHostFree(im->hout);
         if (im->st) (void)hipStreamDestroy(im->st);
         delete im;
     }

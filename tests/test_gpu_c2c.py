@@ -223,6 +223,48 @@ def test_dropin_f64_device_pointers(pkg, dev):
     assert ol.rel_l2(yd.cpu().numpy(), ol.run_oracle("dft", [n], x)) < tol_for("f64", n)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_dropin_host_paths_bitwise(pkg, dev, prec, monkeypatch):
+    """Host buffers reach the same kernels three ways -- 1-D transforms up
+    to 256 KiB run on mapped staging buffers, larger ones and N-D ones go
+    through DMA transfers (NFFT_DROPIN_ZC_MAX=0 forces those) -- and every
+    way gives the device-pointer result bit for bit, in place too."""
+    import torch
+    lib = pkg.dropin(prec)
+    dt = np.float64 if prec == "f64" else np.float32
+    cases = [("dft_1d", (4096,), "dft", 8192, 8192), ("dft_1d", (4096,), "invdft", 8192, 8192),
+             ("dft_1d", (1009,), "dft", 2018, 2018), ("realdft_1d", (8192,), "realdft", 8192, 8194),
+             ("realdft_1d", (8192,), "invrealdft", 8194, 8192), ("t2t3_1d", (2048,), "dct3", 2048, 2048),
+             ("t4_1d", (4096,), "dst4", 4096, 4096), ("dft_2d", (64, 32), "dft", 4096, 4096),
+             ("dft_1d", (1 << 15,), "dft", 1 << 16, 1 << 16)]
+    for i, (mk, dims, op, ni, no) in enumerate(cases):
+        a = getattr(lib, "notorious_fft_mkaux_" + mk)(*dims)
+        assert a
+        fn = getattr(lib, "notorious_fft_" + op)
+        x = ol.uniform(40 + i, ni).astype(dt)
+        xd = torch.from_numpy(x).to(dev)
+        yd = torch.zeros(no, dtype=xd.dtype, device=dev)
+        torch.cuda.synchronize()
+        fn(xd.data_ptr(), yd.data_ptr(), a)
+        want = yd.cpu().numpy()
+        for zc in ("0", "262144"):
+            monkeypatch.setenv("NFFT_DROPIN_ZC_MAX", zc)
+            y = np.full(no, 3.0, dt)
+            fn(x.ctypes.data, y.ctypes.data, a)
+            assert np.array_equal(y, want), (mk, dims, op, zc)
+            if ni == no:
+                w = x.copy()
+                fn(w.ctypes.data, w.ctypes.data, a)
+                assert np.array_equal(w, want), (mk, dims, op, zc, "in place")
+            # host input, device output
+            yd2 = torch.zeros_like(yd)
+            torch.cuda.synchronize()
+            fn(x.ctypes.data, yd2.data_ptr(), a)
+            assert torch.equal(yd2, yd), (mk, dims, op, zc, "device output")
+        monkeypatch.delenv("NFFT_DROPIN_ZC_MAX")
+        lib.notorious_fft_free_aux(a)
+
+
 @pytest.mark.parametrize("side_stream", [False, True])
 def test_dropin_orders_after_queued_writes(pkg, dev, side_stream):
     """The caller queues the write of x (a copy behind a long chain of its own

@@ -128,14 +128,14 @@ def test_dropin_allocation_failures(pkg, dev, monkeypatch):
     print(f"dropin mkaux: {fails} injected failures before success")
     assert fails >= 1
     lib.notorious_fft_free_aux(a)
-    n = 4096
+    n = 1 << 15
     a = lib.notorious_fft_mkaux_dft_1d(n)
     assert a
     x = ol.uniform(33, 2 * n)
     y = np.full_like(x, 7.0)
-    # the first host-pointer call stages through device buffers made on first
-    # use: a failed staging allocation leaves y untouched (the reference's
-    # transforms are void and have no error channel)
+    # the first host-pointer call of 512 KiB stages through device buffers
+    # made on first use: a failed staging allocation leaves y untouched (the
+    # reference's transforms are void and have no error channel)
     arm(monkeypatch, 1)
     lib.notorious_fft_dft(x.ctypes.data, y.ctypes.data, a)
     assert np.all(y == 7.0)
