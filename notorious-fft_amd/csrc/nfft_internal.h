@@ -282,8 +282,9 @@ struct C2C {
 };
 
 // strided complex transform along one axis of a row-major array:
-// element (a, i, b) at a*n*B + i*B + b, in place on `buf`.
+// element (a, i, b) at a*n*B + i*B + b, in place on `buf`, or from `src`
+// (same layout) into `buf` when src is not null.
 hipError_t c2c_axis(const C2C &e, void *buf, long long A, long long B, int dir, double scale, void *scratch,
-                    long long scratch_elems, hipStream_t s);
+                    long long scratch_elems, hipStream_t s, const void *src = nullptr);
 
 }  // namespace nfft

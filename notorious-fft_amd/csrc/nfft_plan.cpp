@@ -1085,20 +1085,26 @@ hipError_t C2C::run(const void *in, void *out, long long nlines, long long in_di
 }
 
 hipError_t c2c_axis(const C2C &e, void *buf, long long A, long long B, int dir, double scale, void *scratch,
-                    long long scratch_elems, hipStream_t s)
+                    long long scratch_elems, hipStream_t s, const void *src)
 {
     const int n = e.n;
+    if (!src) src = buf;
     if (n == 1) {
+        if (src != buf) {
+            const hipError_t err =
+                hipMemcpyAsync(buf, src, (size_t)(A * B) * cx_bytes(e.prec), hipMemcpyDeviceToDevice, s);
+            if (err != hipSuccess) return err;
+        }
         if (scale == 1.0) return hipSuccess;
         return launch_scale_real(buf, 2 * A * B, e.prec, scale, s);
     }
     if (B == 1) {
         const long long fit = e.scratch_per_line ? scratch_elems / e.scratch_per_line : 0;
-        return e.run(buf, buf, A, n, n, dir, scale, scratch, fit, s);
+        return e.run(src, buf, A, n, n, dir, scale, scratch, fit, s);
     }
     if (e.algo == ALG_LINE && n <= col_max(e.prec) && B <= 0x7fffffffLL) {
         ColArgs c{};
-        c.in = buf;
+        c.in = src;
         c.out = buf;
         c.ncols = (int)B;
         c.S = B;
@@ -1113,7 +1119,7 @@ hipError_t c2c_axis(const C2C &e, void *buf, long long A, long long B, int dir, 
     const long long tot = A * n * B;
     if (scratch_elems < tot) return hipErrorInvalidValue;
     const size_t cb = cx_bytes(e.prec);
-    hipError_t err = launch_transpose(buf, scratch, A, n, B, (int)cb, s);
+    hipError_t err = launch_transpose(src, scratch, A, n, B, (int)cb, s);
     if (err != hipSuccess) return err;
     const long long rest = scratch_elems - tot;
     const long long fit = e.scratch_per_line ? rest / e.scratch_per_line : 0;
@@ -1643,12 +1649,13 @@ int nfft_plan::execute(int op, const void *in, void *out, hipStream_t s, int vjp
             const bool vjp_real = op == NFFT_OP_REALDFT;
             const void *src = in;
             if (d > 1) {
+                // the first leading-axis pass reads the input (left intact)
+                // and writes the spectrum copy; the others work in place
                 void *cp = work;
-                e = hipMemcpyAsync(cp, in, (size_t)batch * Pl * H * cb, hipMemcpyDeviceToDevice, s);
                 for (int i = d - 2; i >= 0 && e == hipSuccess; i--) {
                     const long long A = batch * prod(dims, 0, i), B = prod(cd, i + 1, d);
                     const double sc = vjp_real ? 1.0 : axis_inv_scale(norm, dims[i]);
-                    e = c2c_axis(ax[i], cp, A, B, 1, sc, eng, eng_elems, s);
+                    e = c2c_axis(ax[i], cp, A, B, 1, sc, eng, eng_elems, s, i == d - 2 ? in : nullptr);
                 }
                 src = cp;
             }
