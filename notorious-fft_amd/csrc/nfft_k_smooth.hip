@@ -87,8 +87,17 @@ template <typename T> hipError_t smooth4_impl(int dir, const Smooth4Args &a, hip
 }
 }  // namespace
 
+// the standalone kernels run schedules of radices 2-8 only
+static bool small_radices(const SmoothPlan &sp)
+{
+    for (int i = 0; i < sp.ns; i++)
+        if (sp.radix[i] > 8 || sp.radix[i] == 6) return false;
+    return true;
+}
+
 hipError_t launch_smooth4(int prec, int dir, const Smooth4Args &a, hipStream_t s)
 {
+    if (!small_radices(a.sp1) || !small_radices(a.sp2)) return hipErrorInvalidValue;
     const int slot = prof_on() ? prof_begin(s) : -1;
     hipError_t e = prec == F32 ? smooth4_impl<float>(dir, a, s) : smooth4_impl<double>(dir, a, s);
     if (slot >= 0)
@@ -97,7 +106,7 @@ hipError_t launch_smooth4(int prec, int dir, const Smooth4Args &a, hipStream_t s
     return e;
 }
 
-bool smooth_split(long long n, int maxlen, SmoothPlan *sp1, SmoothPlan *sp2)
+bool smooth_split(long long n, int maxlen, SmoothPlan *sp1, SmoothPlan *sp2, bool big)
 {
     // N1 = the divisor of n nearest sqrt(n) with both factors <= maxlen
     long long best = 0;
@@ -108,11 +117,12 @@ bool smooth_split(long long n, int maxlen, SmoothPlan *sp1, SmoothPlan *sp2)
         if (!best || std::llabs(d * d - n) < std::llabs(best * best - n)) best = d;
     }
     if (!best) return false;
-    return smooth_factor(best, sp1) && smooth_factor(n / best, sp2);
+    return smooth_factor(best, sp1, big) && smooth_factor(n / best, sp2, big);
 }
 
 hipError_t launch_smooth_line(int prec, int dir, const SmoothArgs &a, hipStream_t s)
 {
+    if (!small_radices(a.sp)) return hipErrorInvalidValue;
     const int slot = prof_on() ? prof_begin(s) : -1;
     hipError_t e = prec == F32 ? smooth_line_impl<float>(dir, a, s) : smooth_line_impl<double>(dir, a, s);
     if (slot >= 0)
@@ -127,7 +137,7 @@ int smooth_line_max(int prec)
     return prec == F32 ? 2048 : 1024;
 }
 
-bool smooth_factor(long long n, SmoothPlan *sp)
+bool smooth_factor(long long n, SmoothPlan *sp, bool big)
 {
     if (n < 2 || n > (1LL << 30)) return false;
     SmoothPlan p{};
@@ -140,19 +150,56 @@ bool smooth_factor(long long n, SmoothPlan *sp)
         m /= r;
         return true;
     };
-    while (m % 8 == 0)
-        if (!push(8)) return false;
-    while (m % 5 == 0)
-        if (!push(5)) return false;
-    while (m % 4 == 0)
-        if (!push(4)) return false;
-    while (m % 3 == 0)
-        if (!push(3)) return false;
-    while (m % 7 == 0)
-        if (!push(7)) return false;
-    while (m % 2 == 0)
-        if (!push(2)) return false;
-    if (m != 1) return false;
+    const char *env = getenv("NFFT_SMOOTH_RADIX");
+    if (!big || (env && *env && atoi(env) == 0)) {
+        // radices 2-8 only
+        while (m % 8 == 0)
+            if (!push(8)) return false;
+        while (m % 5 == 0)
+            if (!push(5)) return false;
+        while (m % 4 == 0)
+            if (!push(4)) return false;
+        while (m % 3 == 0)
+            if (!push(3)) return false;
+        while (m % 7 == 0)
+            if (!push(7)) return false;
+        while (m % 2 == 0)
+            if (!push(2)) return false;
+        if (m != 1) return false;
+    } else {
+        // fewest stages: 16 and the squares 25, 9 first, then one stage for
+        // the leftover 5, 3 and powers of two together (20, 10, 15, 12, 6)
+        int a = 0, b = 0, c = 0, d = 0;
+        for (; m % 2 == 0; m /= 2) a++;
+        for (; m % 3 == 0; m /= 3) b++;
+        for (; m % 5 == 0; m /= 5) c++;
+        for (; m % 7 == 0; m /= 7) d++;
+        if (m != 1) return false;
+        m = n;
+        for (; a >= 4; a -= 4)
+            if (!push(16)) return false;
+        for (; c >= 2; c -= 2)
+            if (!push(25)) return false;
+        for (; b >= 2; b -= 2)
+            if (!push(9)) return false;
+        if (c == 1) {
+            int r = 5;
+            if (a >= 2) r = 20, a -= 2;
+            else if (a == 1) r = 10, a = 0;
+            else if (b == 1) r = 15, b = 0;
+            if (!push(r)) return false;
+        }
+        if (b == 1) {
+            int r = 3;
+            if (a >= 2) r = 12, a -= 2;
+            else if (a == 1) r = 6, a = 0;
+            if (!push(r)) return false;
+        }
+        if (a > 0 && !push(1 << a)) return false;
+        for (; d > 0; d--)
+            if (!push(7)) return false;
+        if (m != 1) return false;
+    }
     p.ns = ns;
     *sp = p;
     return true;

@@ -104,9 +104,95 @@ __device__ __forceinline__ int fdiv(int n, float inv)
     return (int)((float)n * inv);
 }
 
+// cos and sin of 2 pi e / P, e = 0 .. P-1 (FP64, correctly rounded), for the
+// inner twiddles of the two-level codelets below.
+template <int P> struct RootTab;
+template <> struct RootTab<6> {
+    static constexpr double c[6] = {1.0, 0.5, -0.5, -1.0, -0.5, 0.5};
+    static constexpr double s[6] = {0.0, 0.8660254037844386, 0.8660254037844386, 0.0, -0.8660254037844386, -0.8660254037844386};
+};
+template <> struct RootTab<9> {
+    static constexpr double c[9] = {1.0, 0.766044443118978, 0.17364817766693036, -0.5, -0.9396926207859084, -0.9396926207859084, -0.5, 0.17364817766693036, 0.766044443118978};
+    static constexpr double s[9] = {0.0, 0.6427876096865394, 0.984807753012208, 0.8660254037844386, 0.3420201433256687, -0.3420201433256687, -0.8660254037844386, -0.984807753012208, -0.6427876096865394};
+};
+template <> struct RootTab<10> {
+    static constexpr double c[10] = {1.0, 0.8090169943749475, 0.30901699437494745, -0.30901699437494745, -0.8090169943749475, -1.0, -0.8090169943749475, -0.30901699437494745, 0.30901699437494745, 0.8090169943749475};
+    static constexpr double s[10] = {0.0, 0.5877852522924731, 0.9510565162951535, 0.9510565162951535, 0.5877852522924731, 0.0, -0.5877852522924731, -0.9510565162951535, -0.9510565162951535, -0.5877852522924731};
+};
+template <> struct RootTab<12> {
+    static constexpr double c[12] = {1.0, 0.8660254037844386, 0.5, 0.0, -0.5, -0.8660254037844386, -1.0, -0.8660254037844386, -0.5, 0.0, 0.5, 0.8660254037844386};
+    static constexpr double s[12] = {0.0, 0.5, 0.8660254037844386, 1.0, 0.8660254037844386, 0.5, 0.0, -0.5, -0.8660254037844386, -1.0, -0.8660254037844386, -0.5};
+};
+template <> struct RootTab<15> {
+    static constexpr double c[15] = {1.0, 0.9135454576426009, 0.6691306063588582, 0.30901699437494745, -0.10452846326765347, -0.5, -0.8090169943749475, -0.9781476007338057, -0.9781476007338057, -0.8090169943749475, -0.5, -0.10452846326765347, 0.30901699437494745, 0.6691306063588582, 0.9135454576426009};
+    static constexpr double s[15] = {0.0, 0.4067366430758002, 0.7431448254773942, 0.9510565162951535, 0.9945218953682733, 0.8660254037844386, 0.5877852522924731, 0.20791169081775934, -0.20791169081775934, -0.5877852522924731, -0.8660254037844386, -0.9945218953682733, -0.9510565162951535, -0.7431448254773942, -0.4067366430758002};
+};
+template <> struct RootTab<20> {
+    static constexpr double c[20] = {1.0, 0.9510565162951535, 0.8090169943749475, 0.5877852522924731, 0.30901699437494745, 0.0, -0.30901699437494745, -0.5877852522924731, -0.8090169943749475, -0.9510565162951535, -1.0, -0.9510565162951535, -0.8090169943749475, -0.5877852522924731, -0.30901699437494745, 0.0, 0.30901699437494745, 0.5877852522924731, 0.8090169943749475, 0.9510565162951535};
+    static constexpr double s[20] = {0.0, 0.30901699437494745, 0.5877852522924731, 0.8090169943749475, 0.9510565162951535, 1.0, 0.9510565162951535, 0.8090169943749475, 0.5877852522924731, 0.30901699437494745, 0.0, -0.30901699437494745, -0.5877852522924731, -0.8090169943749475, -0.9510565162951535, -1.0, -0.9510565162951535, -0.8090169943749475, -0.5877852522924731, -0.30901699437494745};
+};
+template <> struct RootTab<25> {
+    static constexpr double c[25] = {1.0, 0.9685831611286311, 0.8763066800438636, 0.7289686274214116, 0.5358267949789967, 0.30901699437494745, 0.06279051952931337, -0.18738131458572463, -0.42577929156507266, -0.6374239897486897, -0.8090169943749475, -0.9297764858882515, -0.9921147013144779, -0.9921147013144779, -0.9297764858882515, -0.8090169943749475, -0.6374239897486897, -0.42577929156507266, -0.18738131458572463, 0.06279051952931337, 0.30901699437494745, 0.5358267949789967, 0.7289686274214116, 0.8763066800438636, 0.9685831611286311};
+    static constexpr double s[25] = {0.0, 0.2486898871648548, 0.48175367410171527, 0.6845471059286887, 0.8443279255020151, 0.9510565162951535, 0.9980267284282716, 0.9822872507286887, 0.9048270524660196, 0.7705132427757893, 0.5877852522924731, 0.368124552684678, 0.12533323356430426, -0.12533323356430426, -0.368124552684678, -0.5877852522924731, -0.7705132427757893, -0.9048270524660196, -0.9822872507286887, -0.9980267284282716, -0.9510565162951535, -0.8443279255020151, -0.6845471059286887, -0.48175367410171527, -0.2486898871648548};
+};
+
+// a * e^{DIR 2 pi i E / P}, E and P compile-time.
+template <int DIR, int P, int E, typename C> __device__ __forceinline__ C twp(C a)
+{
+    constexpr int e = E % P;
+    using R = decltype(C::x);
+    if constexpr (e == 0) return a;
+    else {
+        constexpr double c = RootTab<P>::c[e];
+        constexpr double s = DIR * RootTab<P>::s[e];
+        return cmk<C>(a.x * (R)c - a.y * (R)s, a.x * (R)s + a.y * (R)c);
+    }
+}
+
+template <int R, int DIR, typename C> __device__ __forceinline__ void dft_any(C *v);
+
+// A DFT of P = R1 R2 points in registers, natural order in and out: with
+// n = R2 n1 + n2 and k = k1 + R1 k2, R2 DFTs of length R1 (over n1), the
+// inner twiddles w_P^{n2 k1}, then R1 DFTs of length R2 (over n2).  One stage
+// of radix P replaces a stage of R1 and a stage of R2: half the LDS round
+// trips, index arithmetic and table twiddles of the two.
+template <int R1, int R2, int DIR, typename C> __device__ __forceinline__ void dft_two(C *v)
+{
+    C a[R2][R1];
+    sfor<0, R2>([&](auto n2c) {
+        constexpr int n2 = decltype(n2c)::value;
+        C x[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; n1++) x[n1] = v[R2 * n1 + n2];
+        dft_any<R1, DIR>(x);
+        sfor<0, R1>([&](auto k1c) {
+            constexpr int k1 = decltype(k1c)::value;
+            a[n2][k1] = twp<DIR, R1 * R2, n2 * k1>(x[k1]);
+        });
+    });
+    sfor<0, R1>([&](auto k1c) {
+        constexpr int k1 = decltype(k1c)::value;
+        C y[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; n2++) y[n2] = a[n2][k1];
+        dft_any<R2, DIR>(y);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; k2++) v[k1 + R1 * k2] = y[k2];
+    });
+}
+
+// Radices of the mixed-radix stages: 2, 3, 4, 5, 7, 8, 16, and the two-level
+// 6 = 2 x 3, 9 = 3 x 3, 10 = 2 x 5, 12 = 4 x 3, 15 = 3 x 5, 20 = 4 x 5, 25 = 5 x 5.
 template <int R, int DIR, typename C> __device__ __forceinline__ void dft_any(C *v)
 {
     if constexpr (R == 3 || R == 5 || R == 7) dft_odd<R, DIR>(v);
+    else if constexpr (R == 6) dft_two<2, 3, DIR>(v);
+    else if constexpr (R == 9) dft_two<3, 3, DIR>(v);
+    else if constexpr (R == 10) dft_two<2, 5, DIR>(v);
+    else if constexpr (R == 12) dft_two<4, 3, DIR>(v);
+    else if constexpr (R == 15) dft_two<3, 5, DIR>(v);
+    else if constexpr (R == 20) dft_two<4, 5, DIR>(v);
+    else if constexpr (R == 25) dft_two<5, 5, DIR>(v);
     else dftR<R, DIR>(v);
 }
 
@@ -153,8 +239,11 @@ __device__ __forceinline__ void smooth_stage_any(const Cx<T> *src, Cx<T> *dst, i
 
 // All stages of G lines held in LDS buffer `a` (buffer `b` is the other half
 // of the ping-pong); returns the buffer holding the result.  All threads of
-// the workgroup must call it.
-template <typename T, int DIR>
+// the workgroup must call it.  BIG admits the radices above 8 (schedules of
+// smooth_factor(..., big = true)); without it the kernel keeps the register
+// allocation of the radix-8 butterfly (59-62 VGPRs FP32, 78-92 FP64 against
+// 103 / 170 with them), which its occupancy depends on.
+template <typename T, int DIR, bool BIG = false>
 __device__ __forceinline__ Cx<T> *smooth_lines(Cx<T> *a, Cx<T> *b, int G, const SmoothPlan &sp, const Cx<T> *tw)
 {
     Cx<T> *src = a, *dst = b;
@@ -167,7 +256,21 @@ __device__ __forceinline__ Cx<T> *smooth_lines(Cx<T> *a, Cx<T> *b, int G, const 
         case 4: smooth_stage_any<T, 4, DIR>(src, dst, G, sp.L, Ns, tw); break;
         case 5: smooth_stage_any<T, 5, DIR>(src, dst, G, sp.L, Ns, tw); break;
         case 7: smooth_stage_any<T, 7, DIR>(src, dst, G, sp.L, Ns, tw); break;
-        default: smooth_stage_any<T, 8, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 8: smooth_stage_any<T, 8, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        default:
+            if constexpr (BIG) {
+                switch (r) {
+                case 6: smooth_stage_any<T, 6, DIR>(src, dst, G, sp.L, Ns, tw); break;
+                case 9: smooth_stage_any<T, 9, DIR>(src, dst, G, sp.L, Ns, tw); break;
+                case 10: smooth_stage_any<T, 10, DIR>(src, dst, G, sp.L, Ns, tw); break;
+                case 12: smooth_stage_any<T, 12, DIR>(src, dst, G, sp.L, Ns, tw); break;
+                case 15: smooth_stage_any<T, 15, DIR>(src, dst, G, sp.L, Ns, tw); break;
+                case 16: smooth_stage_any<T, 16, DIR>(src, dst, G, sp.L, Ns, tw); break;
+                case 20: smooth_stage_any<T, 20, DIR>(src, dst, G, sp.L, Ns, tw); break;
+                default: smooth_stage_any<T, 25, DIR>(src, dst, G, sp.L, Ns, tw); break;
+                }
+            }
+            break;
         }
         __syncthreads();
         Cx<T> *t = src;
@@ -244,7 +347,7 @@ namespace nfft {
 // COH selects agent-scope (sc1) accesses for the intermediate (tiles that
 // other workgroups consume within the same launch).
 // ---------------------------------------------------------------------------
-template <typename T, int DIR, bool COH>
+template <typename T, int DIR, bool COH, bool BIG = false>
 __device__ __forceinline__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, const SmoothPlan &sp1, int N2, int c0, int w,
                                 const Cx<T> *tw1, const Tw2 &tw2, Cx<T> *A, Cx<T> *B)
 {
@@ -256,7 +359,7 @@ __device__ __forceinline__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, con
         A[b * N1 + p] = in[(long long)p * N2 + c0 + b];
     }
     __syncthreads();
-    const C *res = smooth_lines<T, DIR>(A, B, w, sp1, tw1);
+    const C *res = smooth_lines<T, DIR, BIG>(A, B, w, sp1, tw1);
     for (int idx = threadIdx.x; idx < w * N1; idx += blockDim.x) {
         const int k1 = fdiv(idx, iw), b = idx - __mul24(k1, w);
         const uint32_t e = (uint32_t)k1 * (uint32_t)(c0 + b);
@@ -268,7 +371,7 @@ __device__ __forceinline__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, con
     __syncthreads();
 }
 
-template <typename T, int DIR, bool COH>
+template <typename T, int DIR, bool COH, bool BIG = false>
 __device__ __forceinline__ void smooth_row_tile(const Cx<T> *scr, Cx<T> *out, const SmoothPlan &sp2, int N1, int r0, int w,
                                 const Cx<T> *tw2t, T scale, Cx<T> *A, Cx<T> *B)
 {
@@ -280,7 +383,7 @@ __device__ __forceinline__ void smooth_row_tile(const Cx<T> *scr, Cx<T> *out, co
         else A[idx] = *s;
     }
     __syncthreads();
-    const C *res = smooth_lines<T, DIR>(A, B, w, sp2, tw2t);
+    const C *res = smooth_lines<T, DIR, BIG>(A, B, w, sp2, tw2t);
     const float iw = rcp_up(w);
     for (int idx = threadIdx.x; idx < w * N2; idx += blockDim.x) {
         const int k2 = fdiv(idx, iw), b = idx - __mul24(k2, w);
