@@ -151,12 +151,15 @@ bool xcd_topology(int *nq, signed char *qmap);
 hipError_t launch_four_xcd(int n1, int dir, bool twrow, const XcdArgs &a, int grid, hipStream_t s);
 
 // mixed-radix lines (nfft_k_smooth.hip); smooth_factor() builds the stage
-// schedule of a 7-smooth length (false otherwise): radices 2-8, or with `big`
-// the fewest stages with radices up to 25 (only for smooth_lines<..., BIG>)
-bool smooth_factor(long long n, SmoothPlan *sp, bool big = false);
+// schedule of a 7-smooth length (false otherwise) with radices up to maxr
+// (8, 10 or 25; the bound smooth_lines<..., MAXR> is compiled with).  The
+// standalone mixed-radix kernels are compiled for smooth_radix<T>()
+// (smooth_radix_of(prec) on the host).
+inline int smooth_radix_of(int prec) { return prec == F32 ? smooth_radix<float>() : smooth_radix<double>(); }
+bool smooth_factor(long long n, SmoothPlan *sp, int maxr);
 hipError_t launch_smooth_line(int prec, int dir, const SmoothArgs &a, hipStream_t s);
 // mixed-radix four-step: n = N1 x N2, both 7-smooth and <= maxlen
-bool smooth_split(long long n, int maxlen, SmoothPlan *sp1, SmoothPlan *sp2, bool big = false);
+bool smooth_split(long long n, int maxlen, SmoothPlan *sp1, SmoothPlan *sp2, int maxr);
 hipError_t launch_smooth4(int prec, int dir, const Smooth4Args &a, hipStream_t s);
 // longest mixed-radix line run by one workgroup (longer 7-smooth lengths
 // use the mixed-radix four-step), and the points per workgroup of the line

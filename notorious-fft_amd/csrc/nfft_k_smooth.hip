@@ -87,17 +87,17 @@ template <typename T> hipError_t smooth4_impl(int dir, const Smooth4Args &a, hip
 }
 }  // namespace
 
-// the standalone kernels run schedules of radices 2-8 only
-static bool small_radices(const SmoothPlan &sp)
+// the standalone kernels run schedules of radices up to smooth_radix<T>() only
+static bool small_radices(const SmoothPlan &sp, int prec)
 {
     for (int i = 0; i < sp.ns; i++)
-        if (sp.radix[i] > 8 || sp.radix[i] == 6) return false;
+        if (sp.radix[i] > smooth_radix_of(prec)) return false;
     return true;
 }
 
 hipError_t launch_smooth4(int prec, int dir, const Smooth4Args &a, hipStream_t s)
 {
-    if (!small_radices(a.sp1) || !small_radices(a.sp2)) return hipErrorInvalidValue;
+    if (!small_radices(a.sp1, prec) || !small_radices(a.sp2, prec)) return hipErrorInvalidValue;
     const int slot = prof_on() ? prof_begin(s) : -1;
     hipError_t e = prec == F32 ? smooth4_impl<float>(dir, a, s) : smooth4_impl<double>(dir, a, s);
     if (slot >= 0)
@@ -106,7 +106,7 @@ hipError_t launch_smooth4(int prec, int dir, const Smooth4Args &a, hipStream_t s
     return e;
 }
 
-bool smooth_split(long long n, int maxlen, SmoothPlan *sp1, SmoothPlan *sp2, bool big)
+bool smooth_split(long long n, int maxlen, SmoothPlan *sp1, SmoothPlan *sp2, int maxr)
 {
     // N1 = the divisor of n nearest sqrt(n) with both factors <= maxlen
     long long best = 0;
@@ -117,12 +117,12 @@ bool smooth_split(long long n, int maxlen, SmoothPlan *sp1, SmoothPlan *sp2, boo
         if (!best || std::llabs(d * d - n) < std::llabs(best * best - n)) best = d;
     }
     if (!best) return false;
-    return smooth_factor(best, sp1, big) && smooth_factor(n / best, sp2, big);
+    return smooth_factor(best, sp1, maxr) && smooth_factor(n / best, sp2, maxr);
 }
 
 hipError_t launch_smooth_line(int prec, int dir, const SmoothArgs &a, hipStream_t s)
 {
-    if (!small_radices(a.sp)) return hipErrorInvalidValue;
+    if (!small_radices(a.sp, prec)) return hipErrorInvalidValue;
     const int slot = prof_on() ? prof_begin(s) : -1;
     hipError_t e = prec == F32 ? smooth_line_impl<float>(dir, a, s) : smooth_line_impl<double>(dir, a, s);
     if (slot >= 0)
@@ -137,69 +137,69 @@ int smooth_line_max(int prec)
     return prec == F32 ? 2048 : 1024;
 }
 
-bool smooth_factor(long long n, SmoothPlan *sp, bool big)
+bool smooth_factor(long long n, SmoothPlan *sp, int maxr)
 {
     if (n < 2 || n > (1LL << 30)) return false;
     SmoothPlan p{};
     p.L = (int)n;
-    long long m = n;
     int ns = 0;
     auto push = [&](int r) {
         if (ns >= kSmoothMaxStages) return false;
         p.radix[ns++] = (unsigned char)r;
-        m /= r;
         return true;
     };
+    int a = 0, b = 0, c = 0, d = 0;
+    long long m = n;
+    for (; m % 2 == 0; m /= 2) a++;
+    for (; m % 3 == 0; m /= 3) b++;
+    for (; m % 5 == 0; m /= 5) c++;
+    for (; m % 7 == 0; m /= 7) d++;
+    if (m != 1) return false;
     const char *env = getenv("NFFT_SMOOTH_RADIX");
-    if (!big || (env && *env && atoi(env) == 0)) {
-        // radices 2-8 only
-        while (m % 8 == 0)
-            if (!push(8)) return false;
-        while (m % 5 == 0)
-            if (!push(5)) return false;
-        while (m % 4 == 0)
-            if (!push(4)) return false;
-        while (m % 3 == 0)
-            if (!push(3)) return false;
-        while (m % 7 == 0)
-            if (!push(7)) return false;
-        while (m % 2 == 0)
-            if (!push(2)) return false;
-        if (m != 1) return false;
-    } else {
+    if (env && *env) maxr = std::min(maxr, std::max(8, atoi(env)));
+    bool ok = true;
+    if (maxr >= 25) {
         // fewest stages: 16 and the squares 25, 9 first, then one stage for
         // the leftover 5, 3 and powers of two together (20, 10, 15, 12, 6)
-        int a = 0, b = 0, c = 0, d = 0;
-        for (; m % 2 == 0; m /= 2) a++;
-        for (; m % 3 == 0; m /= 3) b++;
-        for (; m % 5 == 0; m /= 5) c++;
-        for (; m % 7 == 0; m /= 7) d++;
-        if (m != 1) return false;
-        m = n;
-        for (; a >= 4; a -= 4)
-            if (!push(16)) return false;
-        for (; c >= 2; c -= 2)
-            if (!push(25)) return false;
-        for (; b >= 2; b -= 2)
-            if (!push(9)) return false;
+        for (; a >= 4; a -= 4) ok = ok && push(16);
+        for (; c >= 2; c -= 2) ok = ok && push(25);
+        for (; b >= 2; b -= 2) ok = ok && push(9);
         if (c == 1) {
             int r = 5;
             if (a >= 2) r = 20, a -= 2;
             else if (a == 1) r = 10, a = 0;
             else if (b == 1) r = 15, b = 0;
-            if (!push(r)) return false;
+            ok = ok && push(r);
         }
         if (b == 1) {
             int r = 3;
             if (a >= 2) r = 12, a -= 2;
             else if (a == 1) r = 6, a = 0;
-            if (!push(r)) return false;
+            ok = ok && push(r);
         }
-        if (a > 0 && !push(1 << a)) return false;
-        for (; d > 0; d--)
-            if (!push(7)) return false;
-        if (m != 1) return false;
+        if (a > 0) ok = ok && push(1 << a);
+    } else if (maxr >= 10) {
+        // radices up to 10: each 5 with a 2 (10), 3s in pairs (9) or with a
+        // 2 (6), then 8 / 4 / 2 and the remaining 5s and 3
+        for (; c >= 1 && a >= 1; c--, a--) ok = ok && push(10);
+        for (; b >= 2; b -= 2) ok = ok && push(9);
+        if (b == 1 && a >= 1 && a % 3 != 0) ok = ok && push(6), b = 0, a--;
+        for (; a >= 3; a -= 3) ok = ok && push(8);
+        if (a > 0) ok = ok && push(1 << a);
+        for (; c > 0; c--) ok = ok && push(5);
+        if (b == 1) ok = ok && push(3);
+    } else {
+        // radices 2-8: 8s, 5s, a 4, 3s, 7s, a 2
+        for (; a >= 3; a -= 3) ok = ok && push(8);
+        for (; c > 0; c--) ok = ok && push(5);
+        if (a == 2) ok = ok && push(4);
+        for (; b > 0; b--) ok = ok && push(3);
+        for (; d > 0; d--) ok = ok && push(7);
+        if (a == 1) ok = ok && push(2);
+        a = 0;
     }
+    for (; d > 0; d--) ok = ok && push(7);
+    if (!ok) return false;
     p.ns = ns;
     *sp = p;
     return true;

@@ -21,6 +21,12 @@
 namespace nfft {
 
 constexpr int kSmoothMaxStages = 24;
+// Largest radix of the standalone mixed-radix kernels (k_smooth_line,
+// k_smooth_col, k_smooth_rowT): 10 in FP32 (N = 100 as 10 x 10: 0.78 -> 0.70
+// ms per 2^27 points, N = 10^4 1.85 -> 1.61 ms), 8 in FP64, where the radix-10
+// butterfly's registers (92 -> 118 VGPRs) cost more occupancy than the stages
+// it saves (N = 100 1.21 -> 1.26 ms).
+template <typename T> __host__ __device__ constexpr int smooth_radix() { return sizeof(T) == 4 ? 10 : 8; }
 
 // Stage schedule of a 7-smooth length (radices in execution order).
 struct SmoothPlan {
@@ -239,11 +245,13 @@ __device__ __forceinline__ void smooth_stage_any(const Cx<T> *src, Cx<T> *dst, i
 
 // All stages of G lines held in LDS buffer `a` (buffer `b` is the other half
 // of the ping-pong); returns the buffer holding the result.  All threads of
-// the workgroup must call it.  BIG admits the radices above 8 (schedules of
-// smooth_factor(..., big = true)); without it the kernel keeps the register
-// allocation of the radix-8 butterfly (59-62 VGPRs FP32, 78-92 FP64 against
-// 103 / 170 with them), which its occupancy depends on.
-template <typename T, int DIR, bool BIG = false>
+// the workgroup must call it.  MAXR is the largest radix compiled in: 8 (2-8),
+// 10 (also 6, 9, 10), 25 (also 12, 15, 16, 20, 25); the schedule must come
+// from smooth_factor(..., maxr) with the same bound.  The larger butterflies
+// raise the register allocation of the whole switch (FP32: 59-62 VGPRs at 8,
+// 103 at 25; FP64: 78-92 and 170), which kernels without a fixed register
+// budget pay for in occupancy.
+template <typename T, int DIR, int MAXR = 8>
 __device__ __forceinline__ Cx<T> *smooth_lines(Cx<T> *a, Cx<T> *b, int G, const SmoothPlan &sp, const Cx<T> *tw)
 {
     Cx<T> *src = a, *dst = b;
@@ -257,20 +265,14 @@ __device__ __forceinline__ Cx<T> *smooth_lines(Cx<T> *a, Cx<T> *b, int G, const 
         case 5: smooth_stage_any<T, 5, DIR>(src, dst, G, sp.L, Ns, tw); break;
         case 7: smooth_stage_any<T, 7, DIR>(src, dst, G, sp.L, Ns, tw); break;
         case 8: smooth_stage_any<T, 8, DIR>(src, dst, G, sp.L, Ns, tw); break;
-        default:
-            if constexpr (BIG) {
-                switch (r) {
-                case 6: smooth_stage_any<T, 6, DIR>(src, dst, G, sp.L, Ns, tw); break;
-                case 9: smooth_stage_any<T, 9, DIR>(src, dst, G, sp.L, Ns, tw); break;
-                case 10: smooth_stage_any<T, 10, DIR>(src, dst, G, sp.L, Ns, tw); break;
-                case 12: smooth_stage_any<T, 12, DIR>(src, dst, G, sp.L, Ns, tw); break;
-                case 15: smooth_stage_any<T, 15, DIR>(src, dst, G, sp.L, Ns, tw); break;
-                case 16: smooth_stage_any<T, 16, DIR>(src, dst, G, sp.L, Ns, tw); break;
-                case 20: smooth_stage_any<T, 20, DIR>(src, dst, G, sp.L, Ns, tw); break;
-                default: smooth_stage_any<T, 25, DIR>(src, dst, G, sp.L, Ns, tw); break;
-                }
-            }
-            break;
+        case 6: if constexpr (MAXR >= 10) smooth_stage_any<T, 6, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 9: if constexpr (MAXR >= 10) smooth_stage_any<T, 9, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 10: if constexpr (MAXR >= 10) smooth_stage_any<T, 10, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 12: if constexpr (MAXR >= 25) smooth_stage_any<T, 12, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 15: if constexpr (MAXR >= 25) smooth_stage_any<T, 15, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 16: if constexpr (MAXR >= 25) smooth_stage_any<T, 16, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        case 20: if constexpr (MAXR >= 25) smooth_stage_any<T, 20, DIR>(src, dst, G, sp.L, Ns, tw); break;
+        default: if constexpr (MAXR >= 25) smooth_stage_any<T, 25, DIR>(src, dst, G, sp.L, Ns, tw); break;
         }
         __syncthreads();
         Cx<T> *t = src;
@@ -320,7 +322,7 @@ template <typename T, int DIR, bool TWL = false> __global__ void __launch_bounds
         A[idx] = g < g_here ? src[(l0 + g) * a.in_dist + p] : cmk<C>((T)0, (T)0);
     }
     __syncthreads();
-    const C *res = smooth_lines<T, DIR>(A, B, G, a.sp, twp);
+    const C *res = smooth_lines<T, DIR, smooth_radix<T>()>(A, B, G, a.sp, twp);
     C *dst = (C *)a.out;
     const T sc = (T)a.scale;
     for (int idx = threadIdx.x; idx < G * L; idx += blockDim.x) {
@@ -347,7 +349,7 @@ namespace nfft {
 // COH selects agent-scope (sc1) accesses for the intermediate (tiles that
 // other workgroups consume within the same launch).
 // ---------------------------------------------------------------------------
-template <typename T, int DIR, bool COH, bool BIG = false>
+template <typename T, int DIR, bool COH, int MAXR = 8>
 __device__ __forceinline__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, const SmoothPlan &sp1, int N2, int c0, int w,
                                 const Cx<T> *tw1, const Tw2 &tw2, Cx<T> *A, Cx<T> *B)
 {
@@ -359,7 +361,7 @@ __device__ __forceinline__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, con
         A[b * N1 + p] = in[(long long)p * N2 + c0 + b];
     }
     __syncthreads();
-    const C *res = smooth_lines<T, DIR, BIG>(A, B, w, sp1, tw1);
+    const C *res = smooth_lines<T, DIR, MAXR>(A, B, w, sp1, tw1);
     for (int idx = threadIdx.x; idx < w * N1; idx += blockDim.x) {
         const int k1 = fdiv(idx, iw), b = idx - __mul24(k1, w);
         const uint32_t e = (uint32_t)k1 * (uint32_t)(c0 + b);
@@ -371,7 +373,7 @@ __device__ __forceinline__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, con
     __syncthreads();
 }
 
-template <typename T, int DIR, bool COH, bool BIG = false>
+template <typename T, int DIR, bool COH, int MAXR = 8>
 __device__ __forceinline__ void smooth_row_tile(const Cx<T> *scr, Cx<T> *out, const SmoothPlan &sp2, int N1, int r0, int w,
                                 const Cx<T> *tw2t, T scale, Cx<T> *A, Cx<T> *B)
 {
@@ -383,7 +385,7 @@ __device__ __forceinline__ void smooth_row_tile(const Cx<T> *scr, Cx<T> *out, co
         else A[idx] = *s;
     }
     __syncthreads();
-    const C *res = smooth_lines<T, DIR, BIG>(A, B, w, sp2, tw2t);
+    const C *res = smooth_lines<T, DIR, MAXR>(A, B, w, sp2, tw2t);
     const float iw = rcp_up(w);
     for (int idx = threadIdx.x; idx < w * N2; idx += blockDim.x) {
         const int k2 = fdiv(idx, iw), b = idx - __mul24(k2, w);
@@ -421,7 +423,7 @@ template <typename T, int DIR> __global__ void __launch_bounds__(256) k_smooth_c
     const long long N = (long long)a.sp1.L * N2;
     const int c0 = blockIdx.x * a.w1;
     const int w = (N2 - c0) < a.w1 ? (N2 - c0) : a.w1;
-    smooth_col_tile<T, DIR, false>((const C *)a.in + t * a.in_dist, (C *)a.scratch + t * N, a.sp1, N2, c0, w,
+    smooth_col_tile<T, DIR, false, smooth_radix<T>()>((const C *)a.in + t * a.in_dist, (C *)a.scratch + t * N, a.sp1, N2, c0, w,
                                    (const C *)a.tw1, a.tw2, A, B);
 }
 
@@ -437,7 +439,7 @@ template <typename T, int DIR> __global__ void __launch_bounds__(256) k_smooth_r
     const long long N = (long long)N1 * a.sp2.L;
     const int r0 = blockIdx.x * a.w2;
     const int w = (N1 - r0) < a.w2 ? (N1 - r0) : a.w2;
-    smooth_row_tile<T, DIR, false>((const C *)a.scratch + t * N, (C *)a.out + t * a.out_dist, a.sp2, N1, r0, w,
+    smooth_row_tile<T, DIR, false, smooth_radix<T>()>((const C *)a.scratch + t * N, (C *)a.out + t * a.out_dist, a.sp2, N1, r0, w,
                                    (const C *)a.tw2t, (T)a.scale, A, B);
 }
 

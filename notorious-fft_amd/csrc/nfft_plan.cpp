@@ -708,9 +708,10 @@ bool C2C::init(int n_, int prec_, int smooth_mode, int persist_mode)
     if (smooth_env == 2 && smooth_mode == 2) smooth_mode = 1;
     if (smooth_mode != 0 && smooth_env != 0 && !is_pow2(n)) {
         SmoothPlan p1{}, p2{};
-        const bool line = n <= smooth_line_max(prec) && smooth_factor(n, &p1);
-        const bool four = !line && (smooth_split(n, smooth_line_max(prec), &p1, &p2) ||
-                                    smooth_split(n, prec == F32 ? 8192 : 4096, &p1, &p2));
+        const int mr = smooth_radix_of(prec);
+        const bool line = n <= smooth_line_max(prec) && smooth_factor(n, &p1, mr);
+        const bool four = !line && (smooth_split(n, smooth_line_max(prec), &p1, &p2, mr) ||
+                                    smooth_split(n, prec == F32 ? 8192 : 4096, &p1, &p2, mr));
         if (line || four) {
             const int budget = smooth_tile_points(prec);
             const int w1 = four ? balanced_width(p2.L, budget / p1.L) : 0;

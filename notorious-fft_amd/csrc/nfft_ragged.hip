@@ -363,7 +363,7 @@ __device__ __noinline__ void rg_smooth_group(const RagArgs &a, const RagItem &it
         A[idx] = ((const C *)a.in)[bases[g] + p];
     }
     __syncthreads();
-    const C *res = smooth_lines<T, DIR, true>(A, B, G, tb.sp1, (const C *)tb.tw_a);
+    const C *res = smooth_lines<T, DIR, 25>(A, B, G, tb.sp1, (const C *)tb.tw_a);
     const T sc = DIR > 0 ? (T)1 / (T)L : (T)1;  // inverse: 1/N for non-powers of two
     for (int idx = threadIdx.x; idx < G * L; idx += RG_THREADS) {
         const int g = fdiv(idx, iL), p = idx - __mul24(g, L);
@@ -388,14 +388,14 @@ __device__ __noinline__ void rg_smooth4(const RagArgs &a, const RagItem &it, Cx<
     if (it.phase == 1) {
         for (int g = it.g0; g < it.g1; g++) {
             const int c0 = g * tb.w1, w = (N2 - c0) < tb.w1 ? (N2 - c0) : tb.w1;
-            smooth_col_tile<T, DIR, true, true>((const C *)a.in + base, scr, tb.sp1, N2, c0, w, (const C *)tb.tw_a, tb.tw2,
+            smooth_col_tile<T, DIR, true, 25>((const C *)a.in + base, scr, tb.sp1, N2, c0, w, (const C *)tb.tw_a, tb.tw2,
                                           lds, lds + w * N1);
         }
     } else {
         const T sc = DIR > 0 ? (T)1 / (T)((long long)N1 * N2) : (T)1;
         for (int g = it.g0; g < it.g1; g++) {
             const int r0 = g * tb.w2, w = (N1 - r0) < tb.w2 ? (N1 - r0) : tb.w2;
-            smooth_row_tile<T, DIR, true, true>(scr, (C *)a.out + base, tb.sp2, N1, r0, w, (const C *)tb.tw_b, sc, lds,
+            smooth_row_tile<T, DIR, true, 25>(scr, (C *)a.out + base, tb.sp2, N1, r0, w, (const C *)tb.tw_b, sc, lds,
                                           lds + w * N2);
         }
     }
@@ -695,8 +695,8 @@ static int ragged_create_impl(nfft_ragged **out, int prec, int count, const int 
         }
         RagTab t{};
         SmoothPlan sp1{}, sp2{};
-        const bool smooth_line = !p2 && n <= RG_SMOOTH_MAX && smooth_factor(n, &sp1, true);
-        const bool smooth_four = !p2 && !smooth_line && smooth_split(n, RG_SMOOTH_MAX, &sp1, &sp2, true);
+        const bool smooth_line = !p2 && n <= RG_SMOOTH_MAX && smooth_factor(n, &sp1, 25);
+        const bool smooth_four = !p2 && !smooth_line && smooth_split(n, RG_SMOOTH_MAX, &sp1, &sp2, 25);
         if (smooth_line) {
             size_kind[n] = RG_SMOOTH;
             t.sp1 = sp1;

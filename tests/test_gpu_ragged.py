@@ -29,6 +29,16 @@ def test_ragged_vs_pocketfft(pkg, dev, prec):
     check_vs_pocketfft(pkg, dev, prec, sizes)
 
 
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_ragged_two_level_radices(pkg, dev, prec):
+    """7-smooth lengths whose mixed-radix schedules use every two-level radix
+    (6, 9, 10, 12, 15, 16, 20, 25) and 7, as lines and as four-step tiles,
+    against pocketfft in both directions."""
+    sizes = [6, 9, 10, 12, 15, 16 * 3, 20, 25, 50, 36, 60, 150, 400, 900, 980, 1568, 2025, 3600, 6125,
+             10000, 14400, 20736, 50625, 2 * 3 * 5 * 7 * 16, 7 * 7 * 25 * 16]
+    check_vs_pocketfft(pkg, dev, prec, sizes)
+
+
 @pytest.mark.parametrize("prec,mb,gpt,lag", [("f32", 2, 1, 8), ("f32", 4, 3, 1), ("f64", 4, 2, 8),
                                              ("f64", 2, 64, 0)])
 def test_ragged_slot_reuse(pkg, dev, prec, mb, gpt, lag, monkeypatch):
