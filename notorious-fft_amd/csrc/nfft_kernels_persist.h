@@ -89,6 +89,24 @@ __device__ __forceinline__ Cx<double> coh_load(const Cx<double> *p)
     return cmk<Cx<double>>(__builtin_bit_cast(double, x), __builtin_bit_cast(double, y));
 }
 
+// Agent-scope (sc1) store / load of element idx of a workgroup-uniform base
+// (byte offsets below 2^32).  An FP64 element takes one 16-byte buffer access
+// with the sc1 cache-policy bit, the access an agent-scope atomic of that
+// width would be; the atomic form above splits it into two 8-byte halves, and
+// each half writes through to memory on its own, so every line of an
+// intermediate is written twice.
+constexpr int kCpSc1 = 16;
+__device__ __forceinline__ void coh_st(Cx<float> *base, uint32_t idx, Cx<float> v) { coh_store(base + idx, v); }
+__device__ __forceinline__ void coh_st(Cx<double> *base, uint32_t idx, Cx<double> v)
+{
+    buf_st<Cx<double>, kCpSc1>(buf_rsrc_if(base, true), idx * (uint32_t)sizeof(Cx<double>), 0u, v);
+}
+__device__ __forceinline__ Cx<float> coh_ld(const Cx<float> *base, uint32_t idx) { return coh_load(base + idx); }
+__device__ __forceinline__ Cx<double> coh_ld(const Cx<double> *base, uint32_t idx)
+{
+    return buf_ld<Cx<double>, kCpSc1>(buf_rsrc_if(base, true), idx * (uint32_t)sizeof(Cx<double>), 0u);
+}
+
 // OCC: workgroups per CU the register allocation must allow (1 = no limit
 // beyond one workgroup).
 template <typename T, int N1, int N2, int LPB1, int LPB2, int OCC = 1> struct PShape {
@@ -219,9 +237,8 @@ k_four_persist(PersistArgs a)
             LayCol<LPB1> lay{b};
             line_fft<T, N1, DIR>(reg, lds, lay, j, (const double2 *)a.tw_a);
             apply_tw2_series<DIR>(reg, a.tw2, (uint32_t)c * (uint32_t)j, (uint32_t)c * (uint32_t)S1::NT);
-            C *dst = slot + c;
 #pragma unroll
-            for (int q = 0; q < S1::R; q++) coh_store(dst + (long long)(j + S1::NT * q) * N2, reg[q]);
+            for (int q = 0; q < S1::R; q++) coh_st(slot, (uint32_t)((j + S1::NT * q) * N2 + c), reg[q]);
             // the slot's stores of this wave are complete (acknowledged at
             // agent scope) before the barrier that precedes the counter
             __builtin_amdgcn_s_waitcnt(0);
@@ -229,15 +246,15 @@ k_four_persist(PersistArgs a)
             const int b = threadIdx.x / S2::NT, j = threadIdx.x % S2::NT;
             const int r0 = tile * LPB2;
             C reg[S2::R];
-            const C *src = slot + (long long)r0 * N2;
+            const uint32_t r0off = (uint32_t)r0 * (uint32_t)N2;  // the slot's row r0
             if constexpr (S2::NT >= 64) {
 #pragma unroll
-                for (int q = 0; q < S2::R; q++) reg[q] = coh_load(src + (long long)b * N2 + j + S2::NT * q);
+                for (int q = 0; q < S2::R; q++) reg[q] = coh_ld(slot, r0off + (uint32_t)(b * N2 + j + S2::NT * q));
             } else {  // rows shorter than a wavefront: coalesced copy through LDS
                 constexpr int PADL = lpadlen<T, N2>();
                 for (int idx = threadIdx.x; idx < LPB2 * N2; idx += Sh::BT) {
                     const int bb = idx / N2, p = idx % N2;
-                    lds[bb * PADL + lpad<T>(p)] = coh_load(src + idx);
+                    lds[bb * PADL + lpad<T>(p)] = coh_ld(slot, r0off + (uint32_t)idx);
                 }
                 __syncthreads();
 #pragma unroll

@@ -16,7 +16,7 @@
 // the column / row passes of a four-step for lengths above the LDS limit.
 #pragma once
 #include "nfft_kernels.h"
-#include "nfft_kernels_persist.h"  // coh_load / coh_store
+#include "nfft_kernels_persist.h"  // coh_ld / coh_st
 
 namespace nfft {
 
@@ -366,9 +366,9 @@ __device__ __forceinline__ void smooth_col_tile(const Cx<T> *in, Cx<T> *scr, con
         const int k1 = fdiv(idx, iw), b = idx - __mul24(k1, w);
         const uint32_t e = (uint32_t)k1 * (uint32_t)(c0 + b);
         const C v = ctw64<DIR>(res[b * N1 + k1], tw2_get(tw2, e));
-        C *d = scr + (long long)k1 * N2 + c0 + b;
-        if constexpr (COH) coh_store(d, v);
-        else *d = v;
+        const long long di = (long long)k1 * N2 + c0 + b;
+        if constexpr (COH) coh_st(scr, (uint32_t)di, v);
+        else scr[di] = v;
     }
     __syncthreads();
 }
@@ -380,9 +380,9 @@ __device__ __forceinline__ void smooth_row_tile(const Cx<T> *scr, Cx<T> *out, co
     using C = Cx<T>;
     const int N2 = sp2.L;
     for (int idx = threadIdx.x; idx < w * N2; idx += blockDim.x) {
-        const C *s = scr + (long long)r0 * N2 + idx;
-        if constexpr (COH) A[idx] = coh_load(s);
-        else A[idx] = *s;
+        const long long si = (long long)r0 * N2 + idx;
+        if constexpr (COH) A[idx] = coh_ld(scr, (uint32_t)si);
+        else A[idx] = scr[si];
     }
     __syncthreads();
     const C *res = smooth_lines<T, DIR, MAXR>(A, B, w, sp2, tw2t);

@@ -514,12 +514,13 @@ static FourChoice tune_four(int n, int prec, FourChoice dflt)
 // (wall clock); a healthy wait lasts microseconds.
 static const double kPersistTimeoutSec = 2.0;
 // Where the persistent four-step is the default (NFFT_PERSIST=0/1 forces it
-// off/on): the measured crossover on MI355X is FP32 N <= 2^17 and FP64
-// N <= 2^16; above it the two-launch path over whole-batch chunks is faster.
-// Off by default: the two-launch passes with 1024-point rows measured as
-// fast or faster (FP32 2^15 1.53 vs 1.56 ms, 2^16 1.54 vs 1.55, 2^17 1.64 vs
-// 1.62; FP64 2^14..2^16 3.0 vs 3.5 ms); NFFT_PERSIST=1 selects it.
-static bool persist_default(int, long long) { return false; }
+// off/on).  FP64 N = 2^15 and 2^16: its intermediate goes through one
+// 16-byte agent-scope access per point, and the single launch beats the
+// two-launch path with 1024-point rows (2^27 points: 2^15 1.55 -> 1.38 ms,
+// 2^16 1.55 -> 1.38 ms); from 2^17 its smaller tiles lose (2^17 1.57 vs
+// 1.73 ms, 2^18 1.57 vs 2.28).  FP32 gains nothing over the default paths
+// (2^16 1.53 vs 1.53 ms, 2^17 1.59 vs 1.59, 2^18 1.59 vs 2.40).
+static bool persist_default(int prec, long long n) { return prec == F64 && n >= (1LL << 15) && n <= (1LL << 16); }
 // Intermediate the persistent kernel keeps in flight (slots), sized to stay
 // in the 256 MB Infinity Cache.
 static const long long kPersistSlotBytes = 128LL << 20;

@@ -177,15 +177,16 @@ __device__ __noinline__ void rg_line(const RagArgs &a, const RagItem &it, Cx<T> 
 // ---------------------------------------------------------------------------
 // Scratch accesses of tiles go through agent-scope (sc1) loads and stores,
 // coherent across XCDs without cache maintenance (see nfft_kernels_persist.h).
-template <bool COH, typename C> __device__ __forceinline__ C rg_ld(const C *p)
+// Element i of a workgroup-uniform base (i below 2^28 when COH).
+template <bool COH, typename C> __device__ __forceinline__ C rg_ld(const C *base, long long i)
 {
-    if constexpr (COH) return coh_load(p);
-    else return *p;
+    if constexpr (COH) return coh_ld(base, (uint32_t)i);
+    else return base[i];
 }
-template <bool COH, typename C> __device__ __forceinline__ void rg_st(C *p, C v)
+template <bool COH, typename C> __device__ __forceinline__ void rg_st(C *base, long long i, C v)
 {
-    if constexpr (COH) coh_store(p, v);
-    else *p = v;
+    if constexpr (COH) coh_st(base, (uint32_t)i, v);
+    else base[i] = v;
 }
 
 // column pass over an L1 x L2 matrix (row-major, L2 columns), column groups
@@ -218,7 +219,7 @@ __device__ __forceinline__ void rg_cols(const Cx<T> *src, Cx<T> *dst, const RagT
                 }
                 reg[q] = v;
             } else {
-                reg[q] = rg_ld<MODE == 2>(src + n);
+                reg[q] = rg_ld<MODE == 2>(src, n);
             }
         }
         LayCol<LPB> lay{b};
@@ -237,7 +238,7 @@ __device__ __forceinline__ void rg_cols(const Cx<T> *src, Cx<T> *dst, const RagT
         } else {
             apply_tw2_series<DIR>(reg, tb.tw2, (uint32_t)c * (uint32_t)j, (uint32_t)c * (uint32_t)S::NT);
 #pragma unroll
-            for (int q = 0; q < S::R; q++) rg_st<true>(dst + (j + S::NT * q) * L2 + c, reg[q]);
+            for (int q = 0; q < S::R; q++) rg_st<true>(dst, (j + S::NT * q) * L2 + c, reg[q]);
         }
     }
 }
@@ -259,11 +260,11 @@ __device__ __forceinline__ void rg_rows(const Cx<T> *src, Cx<T> *dst, const RagT
         C reg[S::R];
         if constexpr (S::NT >= 64) {
 #pragma unroll
-            for (int q = 0; q < S::R; q++) reg[q] = rg_ld<true>(src + (long long)r * L2 + j + S::NT * q);
+            for (int q = 0; q < S::R; q++) reg[q] = rg_ld<true>(src, (long long)r * L2 + j + S::NT * q);
         } else {
             for (int idx = threadIdx.x; idx < LPB * L2; idx += RG_THREADS) {
                 const int bb = idx / L2, p = idx % L2;
-                lds[bb * PADL + lpad<T>(p)] = rg_ld<true>(src + (long long)(r0 + bb) * L2 + p);
+                lds[bb * PADL + lpad<T>(p)] = rg_ld<true>(src, (long long)(r0 + bb) * L2 + p);
             }
             __syncthreads();
 #pragma unroll
@@ -290,14 +291,14 @@ __device__ __forceinline__ void rg_rows(const Cx<T> *src, Cx<T> *dst, const RagT
             apply_tw2_series<1>(reg, tb.tw2, (uint32_t)r * (uint32_t)j, (uint32_t)r * (uint32_t)S::NT);
             if constexpr (S::NT >= 64) {
 #pragma unroll
-                for (int q = 0; q < S::R; q++) rg_st<true>(dst + (long long)r * L2 + j + S::NT * q, reg[q]);
+                for (int q = 0; q < S::R; q++) rg_st<true>(dst, (long long)r * L2 + j + S::NT * q, reg[q]);
             } else {
 #pragma unroll
                 for (int q = 0; q < S::R; q++) lds[b * PADL + lpad<T>(j + S::NT * q)] = reg[q];
                 __syncthreads();
                 for (int idx = threadIdx.x; idx < LPB * L2; idx += RG_THREADS) {
                     const int bb = idx / L2, p = idx % L2;
-                    rg_st<true>(dst + (long long)(r0 + bb) * L2 + p, lds[bb * PADL + lpad<T>(p)]);
+                    rg_st<true>(dst, (long long)(r0 + bb) * L2 + p, lds[bb * PADL + lpad<T>(p)]);
                 }
                 __syncthreads();
             }
