@@ -199,8 +199,8 @@ def gauss_mask(dims):
 def masked_power(pkg, torch, out, wanted, with_cpu):
     """Loss sum |W DFT2(x)|^2 and its gradient 2 DFT2^H(W^2 DFT2(x)), 2-D C2C
     2048^2 x 16 with the Gaussian-bump mask: FP32 fused (axis-0 forward,
-    weighting and adjoint in one pass), FP32 unfused, the plain FP64 GPU path
-    (unfused), and the reference on the CPU (dft, weighting, invdft)."""
+    weighting and adjoint in one pass), FP32 unfused, FP64 fused and unfused,
+    and the reference on the CPU (dft, weighting, invdft)."""
     import numpy as np
     dims, batch = [2048, 2048], 16
     p = 2048 * 2048
@@ -208,6 +208,7 @@ def masked_power(pkg, torch, out, wanted, with_cpu):
     dev = torch.device("cuda:0")
     for name, prec, fuse in (("c5_loss_grad_f32_2048^2_x16_fused", "f32", "1"),
                              ("c5_loss_grad_f32_2048^2_x16_unfused", "f32", "0"),
+                             ("c5_loss_grad_f64_2048^2_x16_fused", "f64", "1"),
                              ("c5_loss_grad_f64_2048^2_x16_unfused", "f64", "0")):
         if not wanted(name):
             continue
